@@ -1,0 +1,116 @@
+// demag_plan.hpp -- the demag convolution plan and the fused field epilogue.
+#pragma once
+
+#include "mmx_internal.hpp"
+
+namespace mmx {
+
+// Geometry of one zero-padded convolution, internal (z,y,x) order.
+struct DemagGeom {
+    int n0, n1, n2;    // data
+    int p0, p1, p2;    // padded (engine/zeropad.go:12-22)
+    int h2;            // p2/2 + 1 complex values per row after the R2C pass
+    int Lx;            // p2/2: the R2C pass is a half-length complex FFT
+    int sx;            // row stride (complex) of the spectrum buffers
+    int ko0, ko1;      // octant kernel extents p0/2+1, p1/2+1
+    long N;            // n0 n1 n2
+    long nrows;        // n0 n1
+};
+
+// What the last (C2R) pass does with the demag field of each cell.
+enum EpilogueMode : int {
+    EPI_B = 0,      // out = demag_scale * conv (B in tesla, or H)
+    EPI_HEFF = 1,   // H_eff = demag + exchange + anisotropy + Zeeman
+    EPI_TAU0 = 2,   // first RK stage: tau0 = torque(m, H_eff) (+ Zhang-Li / gamma)
+    EPI_CORR = 3,   // RK corrector: tau1, m_new = norm(m + h/2 (tau1 - tau0)), max|tau1-tau0|
+};
+
+// Everything the fused epilogue needs besides the demag spectrum.
+struct FieldParams {
+    float demag_scale;          // Ms (H) or mu0 Ms (B)
+    float ex_pre;               // 2 Aex / (mu0 Ms)
+    float ex_c[3];              // 1/cell^2, FP64 then rounded (exchange6.cu:152-154)
+    int pbc[3];
+    const float *msat_map;
+    const float *aex_map;
+    int anis;
+    float ku2;                  // 2 Ku / (mu0 Ms)
+    const float *ku_map;
+    float u[3];
+    float hext[3];              // B_ext / mu0
+    float alpha;
+    const float *alpha_map;
+    int zl;                     // Zhang-Li on
+    float zl_pre;               // P muB / (e Ms)
+    float zl_xi;
+    float zl_j[3];              // current density multipliers
+    const float *zl_jmask;      // 3 components or NULL
+    float zl_step[3];           // 1/(12 cell)
+    float inv_gamma;            // weight of zzt in the torque sum
+};
+
+// Per-launch state of the epilogue (pointers that change between calls).
+struct EpilogueIO {
+    const float *m;       // magnetisation the field is evaluated at
+    float *hout;          // optional H_eff / B output
+    float *tout;          // optional torque output (tau/gamma)
+    const float *tau0;    // EPI_CORR: first-stage torque
+    float *mnew;          // EPI_CORR: corrected, normalised m
+    float h;              // EPI_CORR: dt*gamma (float32, solve_rk12.go:153)
+    unsigned int *err;    // EPI_CORR: max |tau1 - tau0| (float bits)
+};
+
+// Tunables of the pass kernels.
+struct DemagTiles {
+    int rows_x;   // rows per workgroup in the x passes
+    int w_y;      // spectrum columns per workgroup in the y passes
+    int w_z;      // spectrum columns per workgroup in the z pass
+};
+
+struct DemagPlanImpl {
+    DemagGeom g{};
+    DemagTiles tiles{};
+    int cell_kernel_type = MMX_KERNEL_ARNE;
+    int accuracy = 6;
+    double cell[3] = {0, 0, 0};
+    int pbc[3] = {0, 0, 0};
+    double bad_over_good = 0;
+    // FFT descriptors (float) for x (half length), y, z
+    LineFFT<float> fx, fy, fz;
+    void *tw_mem = nullptr;      // twiddle tables (device)
+    float2 *xbuf = nullptr;      // 3 x nrows x sx
+    float2 *ybuf = nullptr;      // 3 x n0 x p1 x sx
+    float *kern = nullptr;       // 6 x ko0 x ko1 x h2 (octant, real, 1/Np)
+    float *real_kernel = nullptr;// optional: 6 x p0 p1 p2 real-space kernel
+    hipEvent_t *events = nullptr; // when set: recorded before/after every pass
+    int nevent = 0;
+};
+
+// Pass names in launch order (3-D grids; films run xf, y2d, xi).
+constexpr int kMaxPasses = 5;
+
+// Kernel construction (kernel_build.hip)
+int build_demag_kernel(DemagPlanImpl *P, bool keep_real);
+
+// Launch the five (3-D) or three (2-D) passes; the last one runs `mode`.
+int demag_forward(DemagPlanImpl *P, const float *m, const float *msat_map,
+                  hipStream_t st);
+int demag_forward_pred(DemagPlanImpl *P, const float *m0, const float *tau0,
+                       float h, const float *norm_map, float *mpred,
+                       hipStream_t st);
+int demag_inverse(DemagPlanImpl *P, int mode, const FieldParams &fp,
+                  const EpilogueIO &io, hipStream_t st);
+
+// Standalone local field (no demag) with the same epilogue, for sims without
+// demag and for checks.
+int local_field(const DemagGeom &g, int mode, const FieldParams &fp,
+                const EpilogueIO &io, hipStream_t st);
+
+// Twiddle/plan helpers
+int make_line_fft_f32(int L, LineFFT<float> &f, float2 *tw, float2 *rtw);
+
+} // namespace mmx
+
+struct mmx_demag_plan {
+    mmx::DemagPlanImpl impl;
+};

@@ -1,0 +1,112 @@
+"""Loads libmmx.so and declares the C signatures of include/mmx.h."""
+import ctypes as C
+import os
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libmmx.so")
+
+_vp = C.c_void_p
+_fp = C.c_void_p        # device float* (passed as an integer address)
+_i = C.c_int
+_l = C.c_long
+_f = C.c_float
+_d = C.c_double
+_s = C.c_char_p
+
+# name: (restype, argtypes)
+_SIGS = {
+    "mmx_last_error": (_s, []),
+    "mmx_version": (_s, []),
+    "mmx_init": (_i, [_i]),
+    "mmx_normalize": (_i, [_fp, _fp, _fp, _fp, _i, _vp]),
+    "mmx_torque": (_i, [_fp, _fp, _fp, _fp, _fp, _fp, _fp, _fp, _fp, _fp, _f, _i, _vp]),
+    "mmx_exchange6": (_i, [_fp, _fp, _fp, _fp, _fp, _fp, _fp, _fp, _f, _i, _i, _i, _i, _i, _i,
+                           _f, _f, _f, _vp]),
+    "mmx_uniaxial_anisotropy": (_i, [_fp, _fp, _fp, _fp, _fp, _fp, _fp, _fp, _f, _fp, _f, _fp, _f,
+                                     _fp, _f, _i, _vp]),
+    "mmx_zhangli": (_i, [_fp, _fp, _fp, _fp, _fp, _fp, _fp, _fp, _fp, _fp, _fp, _fp, _fp, _f, _f,
+                         _f, _f, _f, _f, _i, _i, _i, _f, _f, _f, _i, _i, _i, _vp]),
+    "mmx_vec_madd": (_i, [_fp, _fp, _fp, _fp, _fp, _fp, _fp, _fp, _fp, _f, _f, _f, _i, _vp]),
+    "mmx_madd": (_i, [_fp, _fp, _fp, _f, _i, _vp]),
+    "mmx_madd1": (_i, [_fp, _fp, _f, _i, _vp]),
+    "mmx_madd2": (_i, [_fp, _fp, _f, _fp, _f, _i, _vp]),
+    "mmx_mul": (_i, [_fp, _fp, _fp, _i, _vp]),
+    "mmx_reductor_create": (_i, [C.POINTER(_vp), _l]),
+    "mmx_reductor_destroy": (_i, [_vp]),
+    "mmx_reductor_sum": (_i, [_vp, _fp, _l, _vp, C.POINTER(_d)]),
+    "mmx_reductor_dot": (_i, [_vp, _fp, _fp, _l, _vp, C.POINTER(_d)]),
+    "mmx_reductor_max": (_i, [_vp, _fp, _l, _vp, C.POINTER(_f)]),
+    "mmx_reductor_min": (_i, [_vp, _fp, _l, _vp, C.POINTER(_f)]),
+    "mmx_reductor_maxabs": (_i, [_vp, _fp, _l, _vp, C.POINTER(_f)]),
+    "mmx_reductor_maxdiff": (_i, [_vp, _fp, _fp, _l, _vp, C.POINTER(_f)]),
+    "mmx_reductor_maxnorm": (_i, [_vp, _fp, _fp, _fp, _l, _vp, C.POINTER(_f)]),
+    "mmx_demag_plan_create": (_i, [C.POINTER(_vp), _i, _i, _i, _d, _d, _d, _i, _i, _i, _i, _i]),
+    "mmx_demag_plan_destroy": (_i, [_vp]),
+    "mmx_demag_update_b": (_i, [_vp, _fp, _fp, _fp, _fp, _fp, _fp, _fp, _f, _vp]),
+    "mmx_demag_plan_info": (_i, [_vp, C.POINTER(_i), C.POINTER(_i), C.POINTER(_d)]),
+    "mmx_demag_plan_kernel": (_i, [_vp, _vp]),
+    "mmx_demag_plan_real_kernel": (_i, [_vp, _vp]),
+    "mmx_sim_create": (_i, [C.POINTER(_vp), _i, _i, _i, _d, _d, _d, _i, _i, _i]),
+    "mmx_sim_destroy": (_i, [_vp]),
+    "mmx_sim_load": (_i, [_vp, _s]),
+    "mmx_sim_set_kernel": (_i, [_vp, _i, _i]),
+    "mmx_sim_setv": (_i, [_vp, _s, C.POINTER(_d), _i]),
+    "mmx_sim_getv": (_i, [_vp, _s, C.POINTER(_d), _i]),
+    "mmx_sim_setmask": (_i, [_vp, _s, _vp, _i]),
+    "mmx_sim_setarray": (_i, [_vp, _s, _vp, _i]),
+    "mmx_sim_getarray": (_i, [_vp, _s, _vp, _i]),
+    "mmx_sim_device_array": (_i, [_vp, _s, C.POINTER(_vp)]),
+    "mmx_sim_stream": (_i, [_vp, C.POINTER(_vp)]),
+    "mmx_sim_update": (_i, [_vp, _s]),
+    "mmx_sim_step": (_i, [_vp]),
+    "mmx_sim_steps": (_i, [_vp, _i]),
+    "mmx_sim_run": (_i, [_vp, _d]),
+    "mmx_sim_try_step": (_i, [_vp, _d, C.POINTER(_d)]),
+    "mmx_sim_time_passes": (_i, [_vp, _i, C.POINTER(_d), C.POINTER(_i)]),
+}
+
+EXPORTED_SYMBOLS = tuple(sorted(_SIGS))
+
+MMX_KERNEL_ARNE = 0
+MMX_KERNEL_NEWELL = 1
+MMX_KEEP_REAL_KERNEL = 0x100
+
+_ERRNAMES = {1: "MMX_ERR_ARG", 2: "MMX_ERR_SIZE", 3: "MMX_ERR_HIP", 4: "MMX_ERR_NOMEM",
+             5: "MMX_ERR_CONVERGE", 6: "MMX_ERR_NAN", 7: "MMX_ERR_NAME", 8: "MMX_ERR_STATE"}
+
+
+class MMXError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("%s (%d): %s" % (_ERRNAMES.get(code, "?"), code, msg))
+        self.code = code
+
+
+_lib = None
+
+
+def load_library(path=None):
+    """Loads libmmx.so; raises if it is not built (no fallback)."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise ImportError("libmmx.so not built at %s: run __graft_entry__.build() "
+                          "(or make -C 2_amd/csrc) first" % p)
+    L = C.CDLL(p)
+    for name, (res, args) in _SIGS.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    if path is None:
+        _lib = L
+    return L
+
+
+def lib():
+    return load_library()
+
+
+def check(rc):
+    if rc != 0:
+        raise MMXError(rc, lib().mmx_last_error().decode())
+    return rc

@@ -1,0 +1,166 @@
+"""Benchmark: H_eff evaluations/s (+ simulated ns/s) on one MI355X.
+
+Workload: the bulk configuration, 256x256x64 cells of 4 nm (internal order
+64x256x256, padded 128x512x512 demag convolution), Ms = 8e5 A/m,
+A = 1.3e-11 J/m, Ku = 5e5 J/m^3 along z, alpha = 0.1, seeded random unit m,
+FP32.  One "step" = one effective-field evaluation m -> H_eff (demag FFT
+convolution + exchange + anisotropy + Zeeman), inputs resident in HBM.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--rk-steps R] [--no-cpu]
+
+Prints one JSON line.  The roofline object is for the dominant pass, timed
+with HIP events on the stream the passes run on; the cpu_baseline is the
+FP64 CPU oracle (same method, same pass structure) on one evaluation.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for p in (ROOT, os.path.join(ROOT, "2_amd")):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def pass_bytes(n, p):
+    """Algorithmic HBM bytes per launch of each pass (zero padding not
+    counted): inputs and outputs read/written once, 4 B per real, 8 B per
+    complex, 24 B per stored octant kernel point (6 components)."""
+    n0, n1, n2 = n
+    p0, p1, p2 = p
+    h2 = p2 // 2 + 1
+    N = n0 * n1 * n2
+    xrow = 24 * n0 * n1 * h2           # 3 components of the x-transformed rows
+    ycol = 24 * n0 * p1 * h2           # 3 components after the y transform
+    kern = 24 * (p0 // 2 + 1) * (p1 // 2 + 1) * h2
+    if p0 == 1:
+        return {"xf": 12 * N + xrow, "y2d": 2 * xrow + kern, "xi": xrow + 24 * N}
+    return {"xf": 12 * N + xrow, "yf": xrow + ycol, "z": 2 * ycol + kern,
+            "yi": ycol + xrow, "xi": xrow + 24 * N}
+
+
+def cpu_baseline(cfg):
+    """One FP64 H_eff evaluation of the same configuration on the host cores
+    (the oracle: same kernel, same zero-padded FFT convolution, same local
+    terms), kernel construction excluded from the clock."""
+    import numpy as np
+    from oracle import oracle as O
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    prob = O.Problem(cfg["n"], cfg["cell"], msat=cfg["msat"], aex=cfg["aex"],
+                     alpha=cfg["alpha"], bext=cfg["bext"], ku=cfg["ku"], anis_u=cfg["anis_u"])
+    m = prob.normalized(cfg["m"].astype(np.float64).copy())
+    t0 = time.perf_counter()
+    prob.heff(m)
+    dt = time.perf_counter() - t0
+    return {"value": 1.0 / dt, "unit": "H_eff evals/s", "cores": threads, "kind": "port",
+            "sample": "1 FP64 H_eff evaluation of the full 256x256x64 grid (%.1f s), kernel "
+                      "construction excluded" % dt}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--rk-steps", type=int, default=1000)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this build runs on one GPU only (the reference's FFT is single-device, "
+              "fftplanX.go:36-39); --gpus %d is not supported" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import mmx
+    from mmx import synth
+
+    mmx.check(mmx.lib().mmx_init(0))
+    cfg = synth.config(2)
+    sim = mmx.Sim(cfg["n"], cfg["cell"])
+    synth.setup_sim(sim, cfg)
+
+    # warm-up (also builds the demag kernel)
+    t_build = time.perf_counter()
+    sim.update("H_eff")
+    torch.cuda.synchronize()
+    t_build = time.perf_counter() - t_build
+    for _ in range(args.warmup):
+        sim.update("H_eff")
+    torch.cuda.synchronize()
+
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        sim.update("H_eff")
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    ms_per = 1e3 * elapsed / args.steps
+
+    # per-pass clock (HIP events on the sim's stream)
+    passes = sim.time_passes(reps=max(10, min(args.steps, 50)))
+    p = tuple(2 * x if x > 1 else x for x in cfg["n"])
+    nbytes = pass_bytes(cfg["n"], p)
+    dom = max(passes, key=passes.get)
+    achieved = nbytes[dom] / (passes[dom] * 1e-3) / 1e9
+    total_bytes = sum(nbytes.values())
+    eval_gbs = total_bytes / (ms_per * 1e-3) / 1e9
+
+    # adaptive RK12: simulated time per wall time
+    sim_ns = None
+    rk_rate = None
+    if args.rk_steps > 0:
+        sim.setarray("m", cfg["m"])
+        sim.setv("t", 0.0)
+        sim.setv("dt", 1e-15)
+        sim.step()
+        torch.cuda.synchronize()
+        t_start = sim.getv("t")
+        e0 = sim.getv("evals")
+        t0 = time.perf_counter()
+        sim.steps(args.rk_steps)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        sim_ns = (sim.getv("t") - t_start) * 1e9 / wall
+        rk_rate = args.rk_steps / wall
+        evals_per_step = (sim.getv("evals") - e0) / args.rk_steps
+
+    cpu = None if args.no_cpu else cpu_baseline(cfg)
+
+    out = {
+        "metric": "H_eff evals/s + simulated ns/s at 256^2x64 FP32; surrogate vs exact, same GPU",
+        "value": args.steps / elapsed,
+        "unit": "H_eff evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "bulk 256x256x64 cells, 4 nm, Ku along z, seeded random m "
+                               "(internal 64x256x256, padded 128x512x512)",
+                   "steps_are": "H_eff evaluations",
+                   "exact_path_only": True},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+        "cpu_baseline": cpu,
+        "passes_ms": passes,
+        "pass_bytes": nbytes,
+        "eval_algorithmic_GBps": eval_gbs,
+        "eval_hbm_frac": eval_gbs / HBM_PEAK_GBS,
+        "rk12": {"steps": args.rk_steps, "steps_per_s": rk_rate, "simulated_ns_per_s": sim_ns,
+                 "evals_per_step": evals_per_step if args.rk_steps > 0 else None,
+                 "m_maxerror": cfg["maxerr"]},
+        "kernel_build_s": t_build,
+        "surrogate": None,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

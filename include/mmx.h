@@ -1,0 +1,254 @@
+/*
+ * mmx.h -- C ABI of the MI355X (gfx950) micromagnetic solver core.
+ *
+ * Drop-in for the hot path of mumax/2: the libmumax2 kernels that the Go
+ * package mumax/gpu binds through cgo, the cuFFT-based demag convolution
+ * behind MaxwellPlan/FFTPlanX, the Reductor, and the engine-side pieces that
+ * drive them (SumUpdater for H_eff, the torque updaters, solver/rk12).
+ *
+ * Conventions (same as libmumax2: the headers in src/libmumax2/ and src/mumax/gpu/libmumax2.go):
+ *  - arrays are float32 device pointers in the reference's internal (z,y,x)
+ *    order: index i*N1*N2 + j*N2 + k, i = user z (slowest), k = user x;
+ *    component 0 of a vector is the user z component (src/mumax/gpu/doc.go:22-35);
+ *  - a vector field is passed as three component pointers (SoA);
+ *  - a NULL map pointer means "uniform": the map is 1 and the scalar
+ *    multiplier carries the value (src/libmumax2/common_func.h:298-306);
+ *  - kernels run asynchronously on `stream` (a hipStream_t, NULL = the null
+ *    stream); the caller synchronises, as the Go wrappers do with Stream.Sync();
+ *  - the reference is multi-device in form (float** per device); one GPU is
+ *    one device, so these entry points take the single device's pointers;
+ *  - every entry point returns an int status, 0 = MMX_OK.  The reference
+ *    aborts on errors (src/libmumax2/gpu_safe.h:36-41); a caller wanting that
+ *    behaviour aborts on a nonzero status (mmx_last_error() says why).
+ * Only the plan/reductor/sim *_create calls allocate device memory.
+ */
+#ifndef MMX_H
+#define MMX_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MMX_OK 0
+#define MMX_ERR_ARG 1      /* bad argument: sizes, required pointer NULL      */
+#define MMX_ERR_SIZE 2     /* unsupported size (FFT length with prime > 7)    */
+#define MMX_ERR_HIP 3      /* HIP runtime error                               */
+#define MMX_ERR_NOMEM 4    /* device allocation failed                        */
+#define MMX_ERR_CONVERGE 5 /* rk12 cannot converge after maxTry bad steps     */
+#define MMX_ERR_NAN 6      /* NaN/Inf in the error estimate                   */
+#define MMX_ERR_NAME 7     /* unknown quantity or module name                 */
+#define MMX_ERR_STATE 8    /* call not valid in the current state             */
+
+/* Text of the last error on this thread (never NULL). */
+const char *mmx_last_error(void);
+
+/* Library version string. */
+const char *mmx_version(void);
+
+/* Selects the GPU used by later calls; replaces setUsedGPUs
+ * (src/libmumax2/multigpu.cu:11-22) for the one-device case. */
+int mmx_init(int device);
+
+/* ------------------------------------------------------------------------
+ * Field and torque kernels
+ * ---------------------------------------------------------------------- */
+
+/* m <- m/|m|, 0 where norm_map == 0 or |m| == 0.
+ * Replaces normalizeAsync (src/libmumax2/normalize.h, normalize.cu:45-60). */
+int mmx_normalize(float *mx, float *my, float *mz, const float *norm_map,
+                  int N, void *stream);
+
+/* tau/gamma = (m x H - alpha m x (m x H)) / (1 + alpha^2), internal coords.
+ * Replaces torqueAsync (src/libmumax2/torque.h:35, torque.cu:59-78). */
+int mmx_torque(float *tx, float *ty, float *tz, const float *mx,
+               const float *my, const float *mz, const float *hx,
+               const float *hy, const float *hz, const float *alpha_map,
+               float alpha_mul, int N, void *stream);
+
+/* 6-neighbour exchange field, harmonic-mean Aex/Msat weights, clamped
+ * (Neumann) or wrapped edges.  pre = 2 Aex_mul / (mu0 Msat_mul).
+ * Replaces exchange6Async (src/libmumax2/exchange6.h:20, exchange6.cu:145-163). */
+int mmx_exchange6(float *hx, float *hy, float *hz, const float *mx,
+                  const float *my, const float *mz, const float *msat_map,
+                  const float *aex_map, float pre, int N0, int N1, int N2,
+                  int periodic0, int periodic1, int periodic2, float cell0,
+                  float cell1, float cell2, void *stream);
+
+/* H = (2 Ku/(mu0 Ms)) Ku_map/Msat_map (m.u) u.
+ * Replaces uniaxialAnisotropyAsync (src/libmumax2/uniaxialanisotropy.h,
+ * uniaxialanisotropy.cu:234-265). */
+int mmx_uniaxial_anisotropy(float *hx, float *hy, float *hz, const float *mx,
+                            const float *my, const float *mz,
+                            const float *ku_map, const float *msat_map,
+                            float ku2_mu0msat_mul, const float *u_map_x,
+                            float u_mul_x, const float *u_map_y, float u_mul_y,
+                            const float *u_map_z, float u_mul_z, int N,
+                            void *stream);
+
+/* Zhang-Li spin-transfer torque (5-point derivative along the current),
+ * pred = P muB/(e Ms).  One launch for the whole volume.
+ * Replaces zhangli_async (src/libmumax2/zhang-li_torque.h:21-35,
+ * zhang-li_torque.cu:307-429). */
+int mmx_zhangli(float *sttx, float *stty, float *sttz, const float *mx,
+                const float *my, const float *mz, const float *jx_map,
+                const float *jy_map, const float *jz_map,
+                const float *msat_map, const float *pol_map,
+                const float *xi_map, const float *alpha_map, float jmul_x,
+                float jmul_y, float jmul_z, float pred, float xi_mul,
+                float alpha_mul, int N0, int N1, int N2, float cell0,
+                float cell1, float cell2, int periodic0, int periodic1,
+                int periodic2, void *stream);
+
+/* ------------------------------------------------------------------------
+ * Linear algebra used by the sum updaters and the RK stages
+ * ---------------------------------------------------------------------- */
+
+/* dst_c = a_c + mul_c * b_c  (b == NULL: dst_c = a_c + mul_c).
+ * Replaces vecMaddAsync (src/libmumax2/add.h, add.cu:119-150). */
+int mmx_vec_madd(float *dstx, float *dsty, float *dstz, const float *ax,
+                 const float *ay, const float *az, const float *bx,
+                 const float *by, const float *bz, float mulx, float muly,
+                 float mulz, int N, void *stream);
+
+/* dst = a + mul*b  (b == NULL: dst = a + mul).
+ * Replaces maddAsync (src/libmumax2/add.h, add.cu:102-118). */
+int mmx_madd(float *dst, const float *a, const float *b, float mul, int N,
+             void *stream);
+
+/* a += mulb*b  (b == NULL: b = 1).  Replaces madd1Async (add.cu:169-177). */
+int mmx_madd1(float *a, const float *b, float mulb, int N, void *stream);
+
+/* a += mulb*b + mulc*c (NULL: 1).  Replaces madd2Async (add.cu:191-199). */
+int mmx_madd2(float *a, const float *b, float mulb, const float *c,
+              float mulc, int N, void *stream);
+
+/* dst = a*b.  Replaces mulAsync (src/libmumax2/mul.h). */
+int mmx_mul(float *dst, const float *a, const float *b, int N, void *stream);
+
+/* ------------------------------------------------------------------------
+ * Reductor (src/mumax/gpu/reductor.go:19-212 + libmumax2/reduce.cu)
+ * The reference returns float32 results of float32 partial sums; here sums
+ * are accumulated in FP64 and returned as double.  All calls synchronise.
+ * ---------------------------------------------------------------------- */
+typedef struct mmx_reductor mmx_reductor;
+
+/* Reductor.Init: buffers for arrays of up to n floats. */
+int mmx_reductor_create(mmx_reductor **r, long n);
+int mmx_reductor_destroy(mmx_reductor *r);
+/* Reductor.Sum / Dot */
+int mmx_reductor_sum(mmx_reductor *r, const float *a, long n, void *stream,
+                     double *out);
+int mmx_reductor_dot(mmx_reductor *r, const float *a, const float *b, long n,
+                     void *stream, double *out);
+/* Reductor.Max / Min / MaxAbs / MaxDiff */
+int mmx_reductor_max(mmx_reductor *r, const float *a, long n, void *stream,
+                     float *out);
+int mmx_reductor_min(mmx_reductor *r, const float *a, long n, void *stream,
+                     float *out);
+int mmx_reductor_maxabs(mmx_reductor *r, const float *a, long n, void *stream,
+                        float *out);
+int mmx_reductor_maxdiff(mmx_reductor *r, const float *a, const float *b,
+                         long n, void *stream, float *out);
+/* Reductor.MaxNorm: sqrt(max(ax^2+ay^2+az^2)) */
+int mmx_reductor_maxnorm(mmx_reductor *r, const float *ax, const float *ay,
+                         const float *az, long n, void *stream, float *out);
+
+/* ------------------------------------------------------------------------
+ * Demag convolution plan: MaxwellPlan (src/mumax/modules/maxwell_convolution.go)
+ * with its FFTPlanX (src/mumax/gpu/fftplanX.go) and kernel construction
+ * (Kernel_Arne, src/mumax/modules/kernel_magnetostatic.go:30-181).
+ * The plan owns its spectrum buffers and the transformed kernel.
+ * ---------------------------------------------------------------------- */
+typedef struct mmx_demag_plan mmx_demag_plan;
+
+#define MMX_KERNEL_ARNE 0   /* the reference's kernel (accuracy as given)  */
+#define MMX_KERNEL_NEWELL 1 /* Newell tensor, FP64 (no reference counterpart) */
+
+/* MaxwellPlan.init + loadDipoleKernel + LoadKernel.  Builds the kernel in
+ * FP64 on the GPU, transforms it in FP64, stores 6 real symmetric components
+ * (octant) in float32.  Sizes in internal order. */
+int mmx_demag_plan_create(mmx_demag_plan **plan, int n0, int n1, int n2,
+                          double cell0, double cell1, double cell2, int pbc0,
+                          int pbc1, int pbc2, int kernel_type, int accuracy);
+int mmx_demag_plan_destroy(mmx_demag_plan *plan);
+
+/* MaxwellPlan.UpdateB for the demag source: B = mu0 Ms (K * M) with
+ * M = m * msat_map (NULL: 1), Ms = msat_mul.  Output in tesla. */
+int mmx_demag_update_b(mmx_demag_plan *plan, float *bx, float *by, float *bz,
+                       const float *mx, const float *my, const float *mz,
+                       const float *msat_map, float msat_mul, void *stream);
+
+/* Plan geometry: padded size p[3]; octant kernel size ko[3] = (p0/2+1,
+ * p1/2+1, p2/2+1); realness ratio max|Im|/max|Re| of the FFT'd kernel
+ * (the reference panics above 1e-5, maxwell_convolution.go:608-610). */
+int mmx_demag_plan_info(const mmx_demag_plan *plan, int p[3], int ko[3],
+                        double *bad_over_good);
+/* Copy the transformed kernel (6 x ko0*ko1*ko2 float32, components
+ * XX,YY,ZZ,YZ,XZ,XY, already scaled by 1/(p0 p1 p2)) to host memory. */
+int mmx_demag_plan_kernel(const mmx_demag_plan *plan, float *host);
+/* Copy the real-space kernel (6 x p0*p1*p2 float32, same component order,
+ * as Kernel_Arne stores it) to host memory; only valid until the plan's
+ * build scratch is released (flag keep_real_space of the sim, or plans
+ * created with MMX_KEEP_REAL_KERNEL in kernel_type). */
+#define MMX_KEEP_REAL_KERNEL 0x100
+int mmx_demag_plan_real_kernel(const mmx_demag_plan *plan, float *host);
+
+/* ------------------------------------------------------------------------
+ * Simulation engine for this path: the Engine/Quant/module/solver pieces of
+ * src/mumax/engine and src/mumax/modules that drive the kernels above
+ * (micromagnetism = demag + exchange6 + zeeman + llg + solver/rk12 +
+ * maxtorque, mod_micromag.go:22-34; optional anisotropy/uniaxial, zhang-li).
+ * Quantity names are the reference's; values are in internal (z,y,x) order
+ * (the reference's API layer swaps user XYZ, engine/api.go:43-45).
+ * ---------------------------------------------------------------------- */
+typedef struct mmx_sim mmx_sim;
+
+/* setgridsize + setcellsize (+ periodic), load('micromagnetism'). */
+int mmx_sim_create(mmx_sim **sim, int n0, int n1, int n2, double cell0,
+                   double cell1, double cell2, int pbc0, int pbc1, int pbc2);
+int mmx_sim_destroy(mmx_sim *sim);
+/* Engine.LoadModule: "anisotropy/uniaxial", "zhang-li" ("micromagnetism",
+ * "demag", "exchange6", "zeeman", "llg", "solver/rk12", "maxtorque" are
+ * already loaded). */
+int mmx_sim_load(mmx_sim *sim, const char *module);
+/* Demag kernel choice before the first update (default MMX_KERNEL_ARNE, 6). */
+int mmx_sim_set_kernel(mmx_sim *sim, int kernel_type, int accuracy);
+/* setv/getv on VALUE/MASK multipliers: Msat, Aex, alpha, gamma, B_ext[3],
+ * Ku, anisU[3], j[3], polarisation, xi, t, dt, mindt, maxdt, m_maxerror;
+ * read-only: step, badsteps, m_error, m_peakerror, <m>[3], maxtorque,
+ * evals (H_eff evaluations so far). */
+int mmx_sim_setv(mmx_sim *sim, const char *name, const double *v, int ncomp);
+int mmx_sim_getv(mmx_sim *sim, const char *name, double *v, int ncomp);
+/* setmask: spatial maps (host float32, N per component) for Msat, Aex,
+ * alpha, Ku, j (3 comps); NULL host pointer removes the map. */
+int mmx_sim_setmask(mmx_sim *sim, const char *name, const float *host,
+                    int ncomp);
+/* setarray/getarray (host float32, SoA, internal order): m (set/get),
+ * H_eff, torque, B (get; updated first, like Quant.Update). */
+int mmx_sim_setarray(mmx_sim *sim, const char *name, const float *host,
+                     int ncomp);
+int mmx_sim_getarray(mmx_sim *sim, const char *name, float *host, int ncomp);
+/* Device pointer of a quantity's storage (component c at ptr + c*N). */
+int mmx_sim_device_array(mmx_sim *sim, const char *name, float **dev);
+/* The stream all of the sim's work is queued on (hipStream_t). */
+int mmx_sim_stream(mmx_sim *sim, void **stream);
+/* Quant.Update of H_eff (one effective-field evaluation), torque or B.
+ * Asynchronous: queues the kernels and returns. */
+int mmx_sim_update(mmx_sim *sim, const char *name);
+/* Engine.Step / Steps / Run (solver/rk12). */
+int mmx_sim_step(mmx_sim *sim);
+int mmx_sim_steps(mmx_sim *sim, int nsteps);
+int mmx_sim_run(mmx_sim *sim, double duration);
+/* Times the passes of `reps` H_eff evaluations with HIP events on the sim's
+ * stream (no reference counterpart; the benchmark's per-kernel clock).
+ * ms[i] = average ms of pass i; *npass = 5 for 3-D grids (xf, yf, z, yi, xi),
+ * 3 for films (xf, y2d, xi). */
+int mmx_sim_time_passes(mmx_sim *sim, int reps, double *ms, int *npass);
+/* Same-dt replay: one predictor/corrector try with the given dt, no step
+ * control (no reference counterpart; used to compare m(t) step by step). */
+int mmx_sim_try_step(mmx_sim *sim, double dt, double *err);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -670,10 +670,12 @@ void orc_zhangli(const int n[3], const int pbc[3], const double cell[3],
                 if (f1 >= len[d]) f1 = c;
                 if (f2 >= len[d]) f2 = c;
             }
-            b2 = (b2 >= 0) ? b2 : len[d] + b2;
-            b1 = (b1 >= 0) ? b1 : len[d] + b1;
-            f1 = (f1 < len[d]) ? f1 : f1 - len[d];
-            f2 = (f2 < len[d]) ? f2 : f2 - len[d];
+            /* periodic wrap: the reference adds/subtracts len once, which
+             * equals a modulo for len >= 2 */
+            b2 = ((b2 % len[d]) + len[d]) % len[d];
+            b1 = ((b1 % len[d]) + len[d]) % len[d];
+            f1 = f1 % len[d];
+            f2 = f2 % len[d];
             nb[d][0] = b2; nb[d][1] = b1; nb[d][2] = f1; nb[d][3] = f2;
         }
         const double step[3] = {ms0, ms1, ms2};

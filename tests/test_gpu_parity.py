@@ -1,0 +1,332 @@
+"""Parity of the HIP path with the FP64 oracle, through the C ABI.
+
+Tolerances: integer/index work and the Kernel_Arne construction are exact;
+FP32 fields are judged by ||dH||_inf / ||H||_inf <= 1e-5 (the plain-path
+tolerance); m(t) by same-dt replays and free-running <m> traces.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+from oracle import oracle as O  # noqa: E402
+
+TOL = 1e-5
+
+
+def _say(*a):
+    print("[case]", *a, flush=True)
+
+
+@pytest.fixture(scope="module")
+def torch_gpu():
+    import torch
+    assert torch.cuda.is_available(), "GPU tests need a GPU"
+    import mmx
+    mmx.check(mmx.lib().mmx_init(0))
+    return torch
+
+
+def relerr(a, b):
+    return float(np.abs(np.asarray(a, np.float64) - b).max() / max(np.abs(b).max(), 1e-300))
+
+
+# ------------------------------------------------------------ kernel build
+
+ARNE_CASES = [((4, 6, 8), (3e-9, 4e-9, 5e-9), (0, 0, 0)),
+              ((1, 16, 32), (3e-9, 3.90625e-9, 3.90625e-9), (0, 0, 0)),
+              ((6, 8, 10), (2e-9, 2e-9, 2e-9), (0, 0, 0)),
+              ((2, 12, 6), (4e-9, 3e-9, 4e-9), (0, 0, 0))]
+
+
+@pytest.mark.parametrize("n,cell,pbc", ARNE_CASES)
+def test_kernel_arne_bit_exact(torch_gpu, n, cell, pbc):
+    import mmx
+    _say("arne", n, cell)
+    plan = mmx.DemagPlan(n, cell, pbc, keep_real_kernel=True)
+    Kg = plan.real_kernel()
+    Ko = O.kernel_arne(plan.p, cell, pbc)[:6]
+    assert np.array_equal(Kg, Ko), np.abs(Kg - Ko).max()
+
+
+@pytest.mark.parametrize("n,cell,pbc", ARNE_CASES + [((3, 8, 8), (2e-9,) * 3, (0, 1, 0))])
+def test_kernel_fft_octant(torch_gpu, n, cell, pbc):
+    import mmx
+    _say("kernel fft", n, pbc)
+    plan = mmx.DemagPlan(n, cell, pbc, keep_real_kernel=True)
+    p = plan.p
+    K = plan.real_kernel()
+    kh, bad = O.kernel_fft(p, K9=np.concatenate([K, np.zeros((3,) + p, np.float32)]))
+    ko = plan.ko
+    want = kh[:, :ko[0], :ko[1], :ko[2]]
+    got = plan.kernel()
+    assert relerr(got, want) < 1e-6
+    assert plan.bad_over_good < 1e-5
+
+
+# ----------------------------------------------------------- demag field
+
+CONV_CASES = [((1, 32, 128), (3e-9, 3.90625e-9, 3.90625e-9), (0, 0, 0)),
+              ((4, 16, 16), (3e-9, 4e-9, 4e-9), (0, 0, 0)),
+              ((8, 16, 24), (4e-9, 4e-9, 4e-9), (0, 0, 0)),
+              ((3, 10, 14), (2e-9, 3e-9, 5e-9), (0, 0, 0)),
+              ((2, 12, 20), (3e-9, 3e-9, 3e-9), (0, 1, 0)),
+              ((4, 8, 16), (4e-9, 4e-9, 4e-9), (0, 0, 1)),
+              ((1, 7, 9), (5e-9, 3e-9, 3e-9), (0, 0, 0))]
+
+
+@pytest.mark.parametrize("n,cell,pbc", CONV_CASES)
+def test_demag_update_b(torch_gpu, n, cell, pbc):
+    import mmx
+    torch = torch_gpu
+    _say("demag B", n, pbc)
+    rng = np.random.default_rng(sum(n))
+    m = O.normalize(rng.normal(size=(3,) + n)).astype(np.float32)
+    ms_map = rng.uniform(0.5, 1.0, size=n).astype(np.float32)
+    plan = mmx.DemagPlan(n, cell, pbc)
+    p = plan.p
+    K9 = O.kernel_arne(p, cell, pbc)
+    kh, _ = O.kernel_fft(p, K9=K9)
+    for msat_map in (None, ms_map):
+        want = O.MU0 * 8e5 * O.demag_conv(n, p, kh, m.astype(np.float64), msat_map)
+        dm = torch.from_numpy(m).cuda()
+        db = torch.empty_like(dm)
+        dmap = None if msat_map is None else torch.from_numpy(msat_map).cuda()
+        plan.update_b(db.data_ptr(), dm.data_ptr(), 8e5,
+                      None if dmap is None else dmap.data_ptr())
+        torch.cuda.synchronize()
+        assert relerr(db.cpu().numpy(), want) < TOL
+
+
+def test_demag_linearity_full_size(torch_gpu):
+    """256x256x64 (the bulk configuration's grid): B(2a - 3b) = 2B(a) - 3B(b)."""
+    import mmx
+    torch = torch_gpu
+    n = (64, 256, 256)
+    _say("linearity", n)
+    plan = mmx.DemagPlan(n, (4e-9,) * 3)
+    g = torch.Generator(device="cuda")
+    g.manual_seed(7)
+    a = torch.randn((3,) + n, device="cuda", generator=g)
+    b = torch.randn((3,) + n, device="cuda", generator=g)
+    c = 2 * a - 3 * b
+    outs = []
+    for x in (a, b, c):
+        o = torch.empty_like(x)
+        plan.update_b(o.data_ptr(), x.data_ptr(), 8e5)
+        outs.append(o)
+    torch.cuda.synchronize()
+    lhs = outs[2]
+    rhs = 2 * outs[0] - 3 * outs[1]
+    err = (lhs - rhs).abs().max().item() / lhs.abs().max().item()
+    assert err < 1e-5
+
+
+# -------------------------------------------------------- drop-in kernels
+
+def _t(torch, a):
+    return torch.from_numpy(np.ascontiguousarray(a, dtype=np.float32)).cuda()
+
+
+@pytest.mark.parametrize("pbc", [(0, 0, 0), (1, 0, 1)])
+@pytest.mark.parametrize("maps", [False, True])
+def test_exchange6(torch_gpu, pbc, maps):
+    from mmx import kernels as K
+    torch = torch_gpu
+    n = (5, 9, 17)
+    _say("exchange6", pbc, maps)
+    rng = np.random.default_rng(11)
+    m = O.normalize(rng.normal(size=(3,) + n)).astype(np.float32)
+    cell = (3e-9, 4e-9, 5e-9)
+    ms = rng.uniform(0.5, 1.5, n).astype(np.float32) if maps else None
+    ax = rng.uniform(0.5, 1.5, n).astype(np.float32) if maps else None
+    pre = 2 * 1.3e-11 / (O.MU0 * 8e5)
+    cell_f = tuple(float(np.float32(c)) for c in cell)
+    want = O.exchange6(n, pbc, cell_f, float(np.float32(pre)), m, ms, ax)
+    got = K.exchange6(_t(torch, m), pre, cell, pbc,
+                      None if ms is None else _t(torch, ms),
+                      None if ax is None else _t(torch, ax)).cpu().numpy()
+    assert relerr(got, want) < TOL
+
+
+def test_uniaxial_torque_normalize(torch_gpu):
+    from mmx import kernels as K
+    torch = torch_gpu
+    n = (4, 8, 8)
+    _say("uniaxial/torque/normalize")
+    rng = np.random.default_rng(12)
+    m = rng.normal(size=(3,) + n).astype(np.float32)
+    mask = rng.uniform(0, 1, n).astype(np.float32)
+    mask[0, 0, :3] = 0
+    dm = _t(torch, m)
+    K.normalize(dm, _t(torch, mask))
+    want = O.normalize(m.astype(np.float64), mask)
+    assert np.abs(dm.cpu().numpy() - want).max() < 1e-6
+    mn = want.astype(np.float32)
+    u = np.array([0.6, 0.0, 0.8])
+    ku2 = 2 * 5e5 / (O.MU0 * 8e5)
+    got = K.uniaxial_anisotropy(_t(torch, mn), ku2, u).cpu().numpy()
+    assert relerr(got, O.uniaxial(mn, float(np.float32(ku2)), u.astype(np.float32))) < TOL
+    h = rng.normal(size=(3,) + n).astype(np.float32) * 1e5
+    got = K.torque(_t(torch, mn), _t(torch, h), 0.05, _t(torch, mask)).cpu().numpy()
+    want = O.torque(mn, h, float(np.float32(0.05)), mask)
+    assert relerr(got, want) < TOL
+
+
+def test_zhangli_kernel(torch_gpu):
+    from mmx import kernels as K
+    torch = torch_gpu
+    n = (2, 6, 20)
+    _say("zhang-li")
+    rng = np.random.default_rng(13)
+    m = O.normalize(rng.normal(size=(3,) + n)).astype(np.float32)
+    cell = (3e-9, 3.125e-9, 3.125e-9)
+    pred = 0.56 * O.MUB / (O.QE * 8e5)
+    j = (0.0, 0.3e12, 1e12)
+    got = K.zhangli(_t(torch, m), j, pred, 0.05, 0.1, cell).cpu().numpy()
+    want = O.zhangli(n, (0, 0, 0), tuple(float(np.float32(c)) for c in cell), m, j,
+                     float(np.float32(pred)), float(np.float32(0.05)), float(np.float32(0.1)))
+    assert relerr(got, want) < TOL
+
+
+def test_madd_family_exact(torch_gpu):
+    from mmx import kernels as K
+    torch = torch_gpu
+    rng = np.random.default_rng(14)
+    a, b, c = (rng.normal(size=1000).astype(np.float32) for _ in range(3))
+    ta, tb, tc = _t(torch, a), _t(torch, b), _t(torch, c)
+    got = K.madd(ta, tb, 0.25).cpu().numpy()
+    assert np.array_equal(got, a + np.float32(0.25) * b)
+    x = ta.clone()
+    K.madd2(x, tb, 0.5, tc, -0.5)
+    want = a + (np.float32(0.5) * b + np.float32(-0.5) * c)
+    assert np.abs(x.cpu().numpy() - want).max() <= 2e-7 * np.abs(want).max()
+    v = _t(torch, rng.normal(size=(3, 100)))
+    w = _t(torch, rng.normal(size=(3, 100)))
+    got = K.vec_madd(v, w, (1.0, 2.0, 3.0)).cpu().numpy()
+    vv, ww = v.cpu().numpy(), w.cpu().numpy()
+    # a fused multiply-add rounds once: within 1 ulp of the two-rounding value
+    want = vv.astype(np.float64) + np.array([1, 2, 3])[:, None] * ww
+    assert np.abs(got - want).max() <= 1.2e-7 * np.abs(want).max()
+    got = K.vec_madd(v, None, (1.0, 2.0, 3.0)).cpu().numpy()
+    assert np.array_equal(got, vv + np.float32([1, 2, 3])[:, None])
+
+
+def test_reductor(torch_gpu):
+    import mmx
+    torch = torch_gpu
+    rng = np.random.default_rng(15)
+    for n in (1, 7, 1000, 300001):
+        a = rng.normal(size=n).astype(np.float32)
+        b = rng.normal(size=n).astype(np.float32)
+        ta, tb = _t(torch, a), _t(torch, b)
+        r = mmx.Reductor(n)
+        assert abs(r.sum(ta.data_ptr(), n) - a.astype(np.float64).sum()) < 1e-9 * max(1, np.abs(a).sum())
+        assert abs(r.dot(ta.data_ptr(), tb.data_ptr(), n) - (a.astype(np.float64) * b).sum()) < 1e-9 * n
+        assert r.max(ta.data_ptr(), n) == a.max()
+        assert r.min(ta.data_ptr(), n) == a.min()
+        assert r.maxabs(ta.data_ptr(), n) == np.abs(a).max()
+        assert r.maxdiff(ta.data_ptr(), tb.data_ptr(), n) == np.abs(a - b).max()
+
+
+# ------------------------------------------------------------ H_eff, RK12
+
+def _problem_pair(n, cell, anis=False, zl=False, bext=(0.0, 0.0, 0.0), seed=21, alpha=0.02):
+    import mmx
+    rng = np.random.default_rng(seed)
+    m = O.normalize(rng.normal(size=(3,) + n)).astype(np.float32)
+    sim = mmx.Sim(n, cell)
+    sim.setv("Msat", 8e5)
+    sim.setv("Aex", 1.3e-11)
+    sim.setv("alpha", alpha)
+    sim.setv("B_ext", bext)
+    kw = {}
+    if anis:
+        sim.load("anisotropy/uniaxial")
+        sim.setv("Ku", 5e5)
+        sim.setv("anisU", (1.0, 0.0, 0.0))
+        kw.update(ku=5e5, anis_u=(1.0, 0.0, 0.0))
+    if zl:
+        sim.load("zhang-li")
+        sim.setv("j", (0.0, 0.0, 1e12))
+        sim.setv("polarisation", 0.56)
+        sim.setv("xi", 0.05)
+        kw.update(j=(0.0, 0.0, 1e12), pol=0.56, xi=0.05)
+    sim.setarray("m", m)
+    prob = O.Problem(n, cell, msat=8e5, aex=1.3e-11, alpha=alpha, bext=bext, **kw)
+    return sim, prob, m
+
+
+HEFF_CASES = [((1, 32, 128), (3e-9, 3.90625e-9, 3.90625e-9), False, False),
+              ((8, 32, 32), (4e-9, 4e-9, 4e-9), True, False),
+              ((2, 16, 48), (3e-9, 4e-9, 4e-9), False, True)]
+
+
+@pytest.mark.parametrize("n,cell,anis,zl", HEFF_CASES)
+def test_heff_and_torque(torch_gpu, n, cell, anis, zl):
+    _say("heff", n, anis, zl)
+    sim, prob, m = _problem_pair(n, cell, anis, zl, bext=(0.0, 0.004, 0.009))
+    mo = prob.normalized(m.astype(np.float64).copy())
+    want_h = prob.heff(mo)
+    got_h = sim.getarray("H_eff")
+    assert relerr(got_h, want_h) < TOL
+    want_t = prob.torque(mo, want_h)
+    got_t = sim.getarray("torque")
+    assert relerr(got_t, want_t) < TOL
+
+
+def test_rk12_same_dt_replay(torch_gpu):
+    n, cell = (1, 32, 64), (3e-9, 3.90625e-9, 3.90625e-9)
+    _say("rk12 replay", n)
+    sim, prob, m = _problem_pair(n, cell, bext=(0.0, 0.004, 0.009))
+    mo = m.astype(np.float64).copy()
+    for k in range(20):
+        dt = 2e-14 * (1 + 0.1 * k)
+        eg = sim.try_step(dt)
+        eo = prob.try_step(mo, dt)
+        assert abs(eg - eo) <= 1e-3 * eo + 1e-12
+    got = sim.getarray("m")
+    want = prob.normalized(mo)
+    assert np.abs(got - want).max() < 1e-5
+
+
+def test_rk12_free_running_cfg1(torch_gpu):
+    """The film configuration (128x32x1, seeded random m, 10 mT in-plane field),
+    adaptive rk12 at m_maxerror 1e-5 (dt ~ 2 fs): <m>(t) against the oracle
+    for 3 ps (~1700 steps)."""
+    import mmx
+    from mmx import synth
+    cfg = synth.config(1)
+    _say("rk12 free run cfg1")
+    sim = mmx.Sim(cfg["n"], cfg["cell"])
+    synth.setup_sim(sim, cfg)
+    prob = O.Problem(cfg["n"], cfg["cell"], msat=cfg["msat"], aex=cfg["aex"], alpha=cfg["alpha"],
+                     bext=cfg["bext"])
+    prob.set("maxerr", cfg["maxerr"])
+    mo = cfg["m"].astype(np.float64).copy()
+    for t_check in (1e-12, 2e-12, 3e-12):
+        while sim.getv("t") < t_check:
+            sim.step()
+        while prob.get("t") < t_check:
+            prob.step(mo)
+        g = np.array(sim.getv("<m>"))
+        o = prob.average(prob.normalized(mo))
+        # both stop at the first step past t_check (dt ~ 2 fs, <m> moves
+        # ~1e-4 per fs at most here)
+        assert np.abs(g - o).max() < 2e-4, (t_check, g, o)
+
+
+def test_step_counts_and_errors(torch_gpu):
+    import mmx
+    sim = mmx.Sim((1, 8, 8), (3e-9, 5e-9, 5e-9))
+    with pytest.raises(mmx.MMXError):
+        sim.step()                      # Msat not set
+    with pytest.raises(mmx.MMXError):
+        sim.setv("nonsense", 1.0)
+    with pytest.raises(mmx.MMXError):
+        sim.load("nonsense")
+    with pytest.raises(mmx.MMXError):
+        mmx.DemagPlan((1, 8, 11), (1e-9,) * 3)   # padded 22 -> FFT 11: prime > 7
