@@ -61,13 +61,69 @@ def cpu_baseline(cfg):
                       "construction excluded" % dt}
 
 
+def summarize_profiles(kt_db, fetch_db=None, write_db=None):
+    """Markdown summary of rocprofv3 results (rocpd SQLite): per-kernel average
+    duration from a --kernel-trace run and, when given, FETCH_SIZE/WRITE_SIZE
+    per dispatch from --pmc runs, scaled by what the same counters read for the
+    1 GiB copy probe (bench.py --copy-probe)."""
+    import collections
+    import sqlite3
+
+    def short(name):
+        return name.split("(")[0].replace("void ", "")
+
+    c = sqlite3.connect(kt_db)
+    rows = list(c.execute("select name, total_calls, total_duration, average, percentage "
+                          "from top_kernels order by total_duration desc"))
+    lines = ["| kernel | calls | avg duration (us) | share |", "|---|---|---|---|"]
+    for name, calls, tot, avg, pct in rows[:16]:
+        lines.append("| `%s` | %d | %.1f | %.1f%% |" % (short(name), calls, avg, pct))
+    out = "\n".join(lines)
+
+    def counter(db):
+        agg = collections.defaultdict(list)
+        cc = sqlite3.connect(db)
+        for name, val in cc.execute("select k.name, p.counter_value from pmc_events p "
+                                    "join kernels k on k.dispatch_id = p.dispatch_id"):
+            agg[short(name)].append(val * 1024.0)          # counters are in KiB
+        return {k: sum(v) / len(v) for k, v in agg.items()}, \
+            {k: sum(v) for k, v in agg.items()}
+
+    if fetch_db and write_db:
+        fmean, ftot = counter(fetch_db)
+        wmean, wtot = counter(write_db)
+        gib = float(1 << 30)
+        # the copy probe: torch's fill of the 1 GiB source, then the copy (the
+        # runtime's copy kernel, possibly split in several dispatches)
+        copy_name = [k for k in ftot if "copyBuffer" in k]
+        fscale = gib / ftot[copy_name[0]] if copy_name else 1.0
+        wscale = gib / wtot[copy_name[0]] if copy_name else 1.0
+        lines = ["", "Counter scale factors from the 1 GiB copy probe: FETCH_SIZE x %.3f, "
+                 "WRITE_SIZE x %.3f" % (fscale, wscale), "",
+                 "| kernel | FETCH_SIZE (MB/dispatch) | WRITE_SIZE (MB/dispatch) | "
+                 "scaled read+write (MB) |", "|---|---|---|---|"]
+        for k in sorted(fmean, key=lambda k: -fmean[k] - wmean.get(k, 0)):
+            if k.startswith("mmx::"):
+                f, w = fmean[k], wmean.get(k, 0.0)
+                lines.append("| `%s` | %.1f | %.1f | %.1f |" % (k, f / 1e6, w / 1e6,
+                                                                (f * fscale + w * wscale) / 1e6))
+        out += "\n".join(lines)
+    return out
+
+
 def main():
+    if len(sys.argv) > 1 and sys.argv[1] == "--summarize":
+        print(summarize_profiles(*sys.argv[2:5]))
+        return
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--rk-steps", type=int, default=1000)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="also run one 1 GiB device-to-device copy (known bytes) so that "
+                         "profiler byte counters can be calibrated in the same run")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on one GPU only (the reference's FFT is single-device, "
@@ -127,6 +183,18 @@ def main():
         rk_rate = args.rk_steps / wall
         evals_per_step = (sim.getv("evals") - e0) / args.rk_steps
 
+    copy_probe = None
+    if args.copy_probe:
+        src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")   # 1 GiB
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        copy_probe = {"bytes_read": src.numel() * 4, "bytes_written": dst.numel() * 4,
+                      "GBps": 2 * src.numel() * 4 / (time.perf_counter() - t0) / 1e9}
+        del src, dst
+
     cpu = None if args.no_cpu else cpu_baseline(cfg)
 
     out = {
@@ -157,6 +225,7 @@ def main():
                  "evals_per_step": evals_per_step if args.rk_steps > 0 else None,
                  "m_maxerror": cfg["maxerr"]},
         "kernel_build_s": t_build,
+        "copy_probe": copy_probe,
         "surrogate": None,
     }
     print(json.dumps(out))
