@@ -13,6 +13,7 @@
 // Replaces MaxwellPlan.update (maxwell_convolution.go:272-309): zeroing,
 // CopyPad3D, 3 cuFFT R2C, 9 cmadd passes, 3 cuFFT C2R and CopyUnPad3D.
 #include "epilogue.hpp"
+#include "reg_fft.hpp"
 
 namespace mmx {
 
@@ -331,10 +332,17 @@ template <typename K> static void allow_lds(K kernel, size_t bytes)
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
 }
 
+bool fast_length(int L) { return fast_length_impl(L); }
+
 static int y_and_z_passes(DemagPlanImpl *P, hipStream_t st)
 {
     const DemagGeom &g = P->g;
     const int ntx_y = div_up(g.h2, P->tiles.w_y);
+    if (g.p0 == 1 && P->fast_y) {
+        MMX_TRY(fast_y2d(P, st));
+        mark(P, st);
+        return MMX_OK;
+    }
     if (g.p0 == 1) {
         const size_t b = lds_bytes_y(P, 3);
         allow_lds(y2d_kernel, b);
@@ -343,7 +351,10 @@ static int y_and_z_passes(DemagPlanImpl *P, hipStream_t st)
         mark(P, st);
         return MMX_OK;
     }
-    {
+    if (P->fast_y) {
+        MMX_TRY(fast_yf(P, st));
+        mark(P, st);
+    } else {
         const size_t b = lds_bytes_y(P, 1);
         allow_lds(yf_kernel, b);
         yf_kernel<<<dim3(ntx_y, g.n0, 3), dim3(kBlock), b, st>>>(g, P->fy, P->tiles.w_y, P->xbuf,
@@ -351,7 +362,10 @@ static int y_and_z_passes(DemagPlanImpl *P, hipStream_t st)
         MMX_CHECK_LAUNCH();
         mark(P, st);
     }
-    {
+    if (P->fast_z) {
+        MMX_TRY(fast_z(P, st));
+        mark(P, st);
+    } else {
         const int ntx_z = div_up(g.h2, P->tiles.w_z);
         const size_t b = lds_bytes_z(P);
         allow_lds(z_kernel, b);
@@ -360,7 +374,10 @@ static int y_and_z_passes(DemagPlanImpl *P, hipStream_t st)
         MMX_CHECK_LAUNCH();
         mark(P, st);
     }
-    {
+    if (P->fast_y) {
+        MMX_TRY(fast_yi(P, st));
+        mark(P, st);
+    } else {
         const size_t b = lds_bytes_y(P, 1);
         allow_lds(yi_kernel, b);
         yi_kernel<<<dim3(ntx_y, g.n0, 3), dim3(kBlock), b, st>>>(g, P->fy, P->tiles.w_y, P->ybuf,
@@ -378,9 +395,13 @@ int demag_forward(DemagPlanImpl *P, const float *m, const float *msat_map, hipSt
     allow_lds(xf_kernel<0>, b);
     const int nb = div_up(g.nrows, P->tiles.rows_x);
     mark(P, st);
-    xf_kernel<0><<<dim3(nb), dim3(kBlock), b, st>>>(g, P->fx, P->tiles.rows_x, m, msat_map,
-                                                     nullptr, 0.0f, nullptr, nullptr, P->xbuf);
-    MMX_CHECK_LAUNCH();
+    if (P->fast_x) {
+        MMX_TRY(fast_xf(P, 0, m, msat_map, nullptr, 0.0f, nullptr, nullptr, st));
+    } else {
+        xf_kernel<0><<<dim3(nb), dim3(kBlock), b, st>>>(g, P->fx, P->tiles.rows_x, m, msat_map,
+                                                         nullptr, 0.0f, nullptr, nullptr, P->xbuf);
+        MMX_CHECK_LAUNCH();
+    }
     mark(P, st);
     return y_and_z_passes(P, st);
 }
@@ -393,9 +414,15 @@ int demag_forward_pred(DemagPlanImpl *P, const float *m0, const float *tau0, flo
     allow_lds(xf_kernel<1>, b);
     const int nb = div_up(g.nrows, P->tiles.rows_x);
     // the demag source is M = m * Msat_map, norm map = Msat map
-    xf_kernel<1><<<dim3(nb), dim3(kBlock), b, st>>>(g, P->fx, P->tiles.rows_x, m0, norm_map,
-                                                     tau0, h, norm_map, mpred, P->xbuf);
-    MMX_CHECK_LAUNCH();
+    mark(P, st);
+    if (P->fast_x) {
+        MMX_TRY(fast_xf(P, 1, m0, norm_map, tau0, h, norm_map, mpred, st));
+    } else {
+        xf_kernel<1><<<dim3(nb), dim3(kBlock), b, st>>>(g, P->fx, P->tiles.rows_x, m0, norm_map,
+                                                         tau0, h, norm_map, mpred, P->xbuf);
+        MMX_CHECK_LAUNCH();
+    }
+    mark(P, st);
     return y_and_z_passes(P, st);
 }
 
@@ -406,6 +433,11 @@ int demag_inverse(DemagPlanImpl *P, int mode, const FieldParams &fp, const Epilo
     const size_t b = lds_bytes_x(P);
     const int nb = div_up(g.nrows, P->tiles.rows_x);
     const int RB = P->tiles.rows_x;
+    if (P->fast_x) {
+        MMX_TRY(fast_xi(P, mode, fp, io, st));
+        mark(P, st);
+        return MMX_OK;
+    }
     switch (mode) {
     case EPI_B:
         allow_lds(xi_kernel<EPI_B>, b);

@@ -82,6 +82,7 @@ struct DemagPlanImpl {
     float2 *ybuf = nullptr;      // 3 x n0 x p1 x sx
     float *kern = nullptr;       // 6 x ko0 x ko1 x h2 (octant, real, 1/Np)
     float *real_kernel = nullptr;// optional: 6 x p0 p1 p2 real-space kernel
+    bool fast_x = false, fast_y = false, fast_z = false; // register-FFT passes
     hipEvent_t *events = nullptr; // when set: recorded before/after every pass
     int nevent = 0;
 };
@@ -101,13 +102,16 @@ int demag_forward_pred(DemagPlanImpl *P, const float *m0, const float *tau0,
 int demag_inverse(DemagPlanImpl *P, int mode, const FieldParams &fp,
                   const EpilogueIO &io, hipStream_t st);
 
-// Standalone local field (no demag) with the same epilogue, for sims without
-// demag and for checks.
-int local_field(const DemagGeom &g, int mode, const FieldParams &fp,
-                const EpilogueIO &io, hipStream_t st);
-
-// Twiddle/plan helpers
-int make_line_fft_f32(int L, LineFFT<float> &f, float2 *tw, float2 *rtw);
+// Power-of-two passes (demag_fast.hip)
+bool fast_length(int L);
+int fast_xf(const DemagPlanImpl *P, int pred, const float *m, const float *msat_map,
+            const float *tau0, float h, const float *norm_map, float *mpred, hipStream_t st);
+int fast_yf(const DemagPlanImpl *P, hipStream_t st);
+int fast_z(const DemagPlanImpl *P, hipStream_t st);
+int fast_yi(const DemagPlanImpl *P, hipStream_t st);
+int fast_y2d(const DemagPlanImpl *P, hipStream_t st);
+int fast_xi(const DemagPlanImpl *P, int mode, const FieldParams &fp, const EpilogueIO &io,
+            hipStream_t st);
 
 } // namespace mmx
 

@@ -72,7 +72,7 @@ static int plan_init(DemagPlanImpl *P, int n0, int n1, int n2, const double cell
     MMX_REQUIRE(g.p1 >= 2, MMX_ERR_SIZE, "demag plan: user y size must be > 1");
     g.h2 = g.p2 / 2 + 1;
     g.Lx = g.p2 / 2;
-    g.sx = g.h2;
+    g.sx = (g.h2 + 15) / 16 * 16; // 128-byte aligned spectrum rows
     g.ko0 = g.p0 / 2 + 1;
     g.ko1 = g.p1 / 2 + 1;
     g.N = (long)n0 * n1 * n2;
@@ -83,6 +83,9 @@ static int plan_init(DemagPlanImpl *P, int n0, int n1, int n2, const double cell
     }
     P->cell_kernel_type = kernel_type;
     P->accuracy = accuracy;
+    P->fast_x = fast_length(g.Lx);
+    P->fast_y = fast_length(g.p1);
+    P->fast_z = g.p0 > 1 && fast_length(g.p0);
 
     // FFT descriptors
     LineFFT<float> *fs[3] = {&P->fx, &P->fy, &P->fz};

@@ -75,7 +75,15 @@ CONV_CASES = [((1, 32, 128), (3e-9, 3.90625e-9, 3.90625e-9), (0, 0, 0)),
               ((3, 10, 14), (2e-9, 3e-9, 5e-9), (0, 0, 0)),
               ((2, 12, 20), (3e-9, 3e-9, 3e-9), (0, 1, 0)),
               ((4, 8, 16), (4e-9, 4e-9, 4e-9), (0, 0, 1)),
-              ((1, 7, 9), (5e-9, 3e-9, 3e-9), (0, 0, 0))]
+              ((1, 7, 9), (5e-9, 3e-9, 3e-9), (0, 0, 0)),
+              # power-of-two transforms (register-FFT passes), incl. mixed paths
+              ((16, 64, 64), (3e-9, 3e-9, 3e-9), (0, 0, 0)),
+              ((1, 256, 512), (3e-9, 4e-9, 4e-9), (0, 0, 0)),
+              ((32, 128, 256), (4e-9, 4e-9, 4e-9), (0, 0, 0)),
+              ((4, 48, 64), (3e-9, 4e-9, 4e-9), (0, 0, 0)),
+              ((6, 32, 40), (3e-9, 4e-9, 4e-9), (0, 0, 0)),
+              ((2, 16, 512), (3e-9, 4e-9, 4e-9), (0, 0, 0)),
+              ((1, 1024, 8), (3e-9, 4e-9, 4e-9), (0, 0, 0))]
 
 
 @pytest.mark.parametrize("n,cell,pbc", CONV_CASES)
