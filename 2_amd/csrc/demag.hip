@@ -17,41 +17,6 @@
 
 namespace mmx {
 
-// ------------------------------------------------------------- kernel values
-
-struct Kern6 {
-    float xx, yy, zz, yz, xz, xy;
-};
-
-// Octant storage: K(kz,ky,kx) for kz <= p0/2, ky <= p1/2; the rest follows
-// from K_sd being even in every direction except s and d, where it is odd.
-__device__ inline Kern6 kern_at(const DemagGeom &g, const float *__restrict__ K, int kz,
-                                int ky, int kx)
-{
-    const bool mz = kz > g.p0 / 2, my = ky > g.p1 / 2;
-    const int kzp = mz ? g.p0 - kz : kz;
-    const int kyp = my ? g.p1 - ky : ky;
-    const float sz = mz ? -1.0f : 1.0f, sy = my ? -1.0f : 1.0f;
-    const long plane = (long)g.ko0 * g.ko1 * g.h2;
-    const long o = ((long)kzp * g.ko1 + kyp) * g.h2 + kx;
-    Kern6 r;
-    r.xx = K[o];
-    r.yy = K[plane + o];
-    r.zz = K[2 * plane + o];
-    r.yz = sy * K[3 * plane + o];
-    r.xz = sz * K[4 * plane + o];
-    r.xy = sz * sy * K[5 * plane + o];
-    return r;
-}
-
-__device__ inline void kern_apply(const Kern6 &k, float2 &a0, float2 &a1, float2 &a2)
-{
-    const float2 s0 = a0, s1 = a1, s2 = a2;
-    a0 = float2{k.xx * s0.x + k.xy * s1.x + k.xz * s2.x, k.xx * s0.y + k.xy * s1.y + k.xz * s2.y};
-    a1 = float2{k.xy * s0.x + k.yy * s1.x + k.yz * s2.x, k.xy * s0.y + k.yy * s1.y + k.yz * s2.y};
-    a2 = float2{k.xz * s0.x + k.yz * s1.x + k.zz * s2.x, k.xz * s0.y + k.yz * s1.y + k.zz * s2.y};
-}
-
 // ------------------------------------------------------------------ XF pass
 
 template <int PRED>
@@ -433,7 +398,7 @@ int demag_inverse(DemagPlanImpl *P, int mode, const FieldParams &fp, const Epilo
     const size_t b = lds_bytes_x(P);
     const int nb = div_up(g.nrows, P->tiles.rows_x);
     const int RB = P->tiles.rows_x;
-    if (P->fast_x) {
+    if (P->fast_x && fast_xi_applies(fp)) {
         MMX_TRY(fast_xi(P, mode, fp, io, st));
         mark(P, st);
         return MMX_OK;

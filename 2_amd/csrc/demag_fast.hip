@@ -29,86 +29,57 @@ template <int LX> struct RowTile {
     static constexpr int E = Split<LX>::E;
     static constexpr int G = E / T;
     static constexpr int R = (LX == 2048) ? 4 : (LX == 1024 ? 8 : 256 / T);
-    static constexpr int S = T * R + 8;
+    // pitch = 1 mod 32 complex: the transposed reads of 16-32 rows x T lanes
+    // hit distinct bank pairs
+    static constexpr int S = T * R + 1;
     static constexpr int NT = T * R;
     static constexpr int LDS = (E * S + 2 * LX + 1) * 8;
 };
-
-struct Kern6f {
-    float xx, yy, zz, yz, xz, xy;
-};
-
-__device__ __forceinline__ Kern6f kern6_at(const DemagGeom &g, const float *__restrict__ K, int kz,
-                                           int ky, int kx)
-{
-    const bool mz = kz > g.p0 / 2, my = ky > g.p1 / 2;
-    const int kzp = mz ? g.p0 - kz : kz;
-    const int kyp = my ? g.p1 - ky : ky;
-    const float sz = mz ? -1.0f : 1.0f, sy = my ? -1.0f : 1.0f;
-    const long plane = (long)g.ko0 * g.ko1 * g.h2;
-    const long o = ((long)kzp * g.ko1 + kyp) * g.h2 + kx;
-    Kern6f r;
-    r.xx = K[o];
-    r.yy = K[plane + o];
-    r.zz = K[2 * plane + o];
-    r.yz = sy * K[3 * plane + o];
-    r.xz = sz * K[4 * plane + o];
-    r.xy = sz * sy * K[5 * plane + o];
-    return r;
-}
-
-__device__ __forceinline__ void kern6_apply(const Kern6f &k, float2 &a0, float2 &a1, float2 &a2)
-{
-    const float2 s0 = a0, s1 = a1, s2 = a2;
-    a0 = float2{k.xx * s0.x + k.xy * s1.x + k.xz * s2.x, k.xx * s0.y + k.xy * s1.y + k.xz * s2.y};
-    a1 = float2{k.xy * s0.x + k.yy * s1.x + k.yz * s2.x, k.xy * s0.y + k.yy * s1.y + k.yz * s2.y};
-    a2 = float2{k.xz * s0.x + k.yz * s1.x + k.zz * s2.x, k.xz * s0.y + k.yz * s1.y + k.zz * s2.y};
-}
 
 // ---------------------------------------------------- column transforms
 //
 // Thread (w, t) of a column tile: w = tid % W (column), t = tid / W.
 // Transpose buffer index of Y[t][k1] of column w: k1*S + t*W + w.
 
-template <int L>
-__device__ __forceinline__ void col_forward(float2 (&v)[ColTile<L>::E],
-                                            float2 (&u)[ColTile<L>::G][ColTile<L>::T], int w,
-                                            int t, float2 *sbuf, const float2 *stw)
+template <int L, int W, int S>
+__device__ __forceinline__ void col_forward(float2 (&v)[Split<L>::E],
+                                            float2 (&u)[Split<L>::E / Split<L>::T][Split<L>::T],
+                                            int w, int t, float2 *sbuf, const float2 *stw)
 {
-    using C = ColTile<L>;
+    constexpr int T = Split<L>::T, E = Split<L>::E, G = E / T;
     fs_first<L, -1>(v, t, stw);
 #pragma unroll
-    for (int k1 = 0; k1 < C::E; k1++) sbuf[k1 * C::S + t * C::W + w] = v[k1];
+    for (int k1 = 0; k1 < E; k1++) sbuf[k1 * S + t * W + w] = v[k1];
     __syncthreads();
 #pragma unroll
-    for (int gg = 0; gg < C::G; gg++) {
+    for (int gg = 0; gg < G; gg++) {
 #pragma unroll
-        for (int t2 = 0; t2 < C::T; t2++) u[gg][t2] = sbuf[(t * C::G + gg) * C::S + t2 * C::W + w];
-        fft_reg<C::T, -1>(u[gg]);
+        for (int t2 = 0; t2 < T; t2++) u[gg][t2] = sbuf[(t * G + gg) * S + t2 * W + w];
+        fft_reg<T, -1>(u[gg]);
     }
     __syncthreads();
 }
 
-template <int L>
-__device__ __forceinline__ void col_inverse(float2 (&u)[ColTile<L>::G][ColTile<L>::T],
-                                            float2 (&v)[ColTile<L>::E], int w, int t,
+template <int L, int W, int S>
+__device__ __forceinline__ void col_inverse(float2 (&u)[Split<L>::E / Split<L>::T][Split<L>::T],
+                                            float2 (&v)[Split<L>::E], int w, int t,
                                             float2 *sbuf, const float2 *stw)
 {
-    using C = ColTile<L>;
+    constexpr int T = Split<L>::T, E = Split<L>::E, G = E / T;
 #pragma unroll
-    for (int gg = 0; gg < C::G; gg++) {
-        fft_reg<C::T, +1>(u[gg]);
-        const int k1 = t * C::G + gg;
+    for (int gg = 0; gg < G; gg++) {
+        fft_reg<T, +1>(u[gg]);
+        const int k1 = t * G + gg;
 #pragma unroll
-        for (int t2 = 0; t2 < C::T; t2++) {
+        for (int t2 = 0; t2 < T; t2++) {
             const float2 x = (t2 == 0) ? u[gg][0] : cmulc(u[gg][t2], stw[t2 * k1]);
-            sbuf[k1 * C::S + t2 * C::W + w] = x;
+            sbuf[k1 * S + t2 * W + w] = x;
         }
     }
     __syncthreads();
 #pragma unroll
-    for (int k1 = 0; k1 < C::E; k1++) v[k1] = sbuf[k1 * C::S + t * C::W + w];
-    fft_reg<C::E, +1>(v);
+    for (int k1 = 0; k1 < E; k1++) v[k1] = sbuf[k1 * S + t * W + w];
+    fft_reg<E, +1>(v);
     __syncthreads();
 }
 
@@ -136,7 +107,7 @@ yf_fast(DemagGeom g, int ntx, const float2 *__restrict__ tw, const float2 *__res
     }
     __syncthreads();
     float2 u[C::G][C::T];
-    col_forward<L>(v, u, w, t, sbuf, stw);
+    col_forward<L, C::W, C::S>(v, u, w, t, sbuf, stw);
     if (!ok) return;
     float2 *dst = yb + ((long)c * g.n0 + i) * (long)g.p1 * g.sx + kx0 + w;
 #pragma unroll
@@ -169,7 +140,7 @@ yi_fast(DemagGeom g, int ntx, const float2 *__restrict__ tw, const float2 *__res
             u[gg][k2] = ok ? src[(long)(t * C::G + gg + C::E * k2) * g.sx] : float2{0.0f, 0.0f};
     __syncthreads();
     float2 v[C::E];
-    col_inverse<L>(u, v, w, t, sbuf, stw);
+    col_inverse<L, C::W, C::S>(u, v, w, t, sbuf, stw);
     if (!ok) return;
     float2 *dst = xb + ((long)c * g.nrows + (long)i * g.n1) * g.sx + kx0 + w;
 #pragma unroll
@@ -179,54 +150,100 @@ yi_fast(DemagGeom g, int ntx, const float2 *__restrict__ tw, const float2 *__res
     }
 }
 
-// z: FFT along z (planes i >= n0 are zeros), kernel multiply, inverse, keep i < n0
+// z pass: three thread groups, one per field component, each transforms its
+// component of W spectrum columns along z; the 3x3 kernel multiply runs
+// through LDS, one octant kernel point for each pair kz, p0-kz.
+template <int L> struct ZTile {
+    static constexpr int T = Split<L>::T;
+    static constexpr int E = Split<L>::E;
+    static constexpr int G = E / T;
+    static constexpr int W = (L == 2048) ? 2 : (128 / T);
+    static constexpr int S = T * W + 8;
+    static constexpr int NG = T * W;          // threads per component group
+    static constexpr int NT = 3 * NG;
+    static constexpr int LDS = (3 * E * S + L) * 8;
+};
+
+// ky visiting order 0, p/2, 1, p-1, 2, p-2, ...: tiles of ky and p - ky
+// (which read the same kernel octant rows) are neighbours
+__device__ __forceinline__ int mirror_order(int i, int p)
+{
+    if (i == 0) return 0;
+    if (p & 1) { // odd p: pairs (u, p-u) for u = 1 .. (p-1)/2
+        const int u = (i - 1) / 2 + 1;
+        return (i & 1) ? u : p - u;
+    }
+    if (i == 1) return p / 2;
+    const int u = (i - 2) / 2 + 1;
+    return (i & 1) ? p - u : u;
+}
+
 template <int L>
-__global__ void __launch_bounds__(ColTile<L>::NT)
+__global__ void __launch_bounds__(ZTile<L>::NT)
 z_fast(DemagGeom g, int ntx, const float2 *__restrict__ tw, const float *__restrict__ K,
        float2 *__restrict__ yb)
 {
-    using C = ColTile<L>;
+    using Z = ZTile<L>;
     extern __shared__ float2 sm[];
-    float2 *sbuf = sm, *stw = sm + C::E * C::S;
+    float2 *stw = sm + 3 * Z::E * Z::S;
     const int tile = xcd_tile(blockIdx.x, gridDim.x);
-    const int tx = tile % ntx, ky = tile / ntx;
-    const int kx0 = tx * C::W;
-    const int w = threadIdx.x % C::W, t = threadIdx.x / C::W;
-    for (int q = threadIdx.x; q < L; q += C::NT) stw[q] = tw[q];
+    const int tx = tile % ntx, ky = mirror_order(tile / ntx, g.p1);
+    const int kx0 = tx * Z::W;
+    const int c = threadIdx.x / Z::NG, lt = threadIdx.x % Z::NG;
+    const int w = lt % Z::W, t = lt / Z::W;
+    float2 *sbuf = sm + c * Z::E * Z::S;
+    for (int q = threadIdx.x; q < L; q += Z::NT) stw[q] = tw[q];
     const bool ok = kx0 + w < g.h2;
-    const long cstride = (long)g.n0 * g.p1 * g.sx, istride = (long)g.p1 * g.sx;
-    float2 *base = yb + (long)ky * g.sx + kx0 + w;
-    float2 u[3][C::G][C::T];
+    const long istride = (long)g.p1 * g.sx;
+    float2 *base = yb + (long)c * g.n0 * istride + (long)ky * g.sx + kx0 + w;
+    float2 v[Z::E];
+#pragma unroll
+    for (int q = 0; q < Z::E; q++) {
+        const int i = t + Z::T * q;
+        v[q] = (ok && i < g.n0) ? base[i * istride] : float2{0.0f, 0.0f};
+    }
+    __syncthreads();
+    float2 u[Z::G][Z::T];
+    col_forward<L, Z::W, Z::S>(v, u, w, t, sbuf, stw);
+
+    // spectrum of all three components through LDS: spec[c][kz][w]
+#pragma unroll
+    for (int gg = 0; gg < Z::G; gg++)
+#pragma unroll
+        for (int k2 = 0; k2 < Z::T; k2++)
+            sm[(c * L + t * Z::G + gg + Z::E * k2) * Z::W + w] = u[gg][k2];
+    __syncthreads();
+    const bool my = ky > g.p1 / 2;
+    const int kyp = my ? g.p1 - ky : ky;
+    const float sy = my ? -1.0f : 1.0f;
+    for (int item = threadIdx.x; item < (L / 2 + 1) * Z::W; item += Z::NT) {
+        const int w2 = item % Z::W, kzp = item / Z::W;
+        if (kx0 + w2 >= g.h2) continue;
+        const Kern6 k = kern_octant(g, K, kzp, kyp, kx0 + w2, 1.0f, sy);
+        kern_apply(k, sm[(0 * L + kzp) * Z::W + w2], sm[(1 * L + kzp) * Z::W + w2],
+                   sm[(2 * L + kzp) * Z::W + w2]);
+        if (kzp > 0 && kzp < L / 2) {
+            Kern6 km = k; // K at p0 - kz: odd in z for XZ and XY
+            km.xz = -km.xz;
+            km.xy = -km.xy;
+            const int kz2 = L - kzp;
+            kern_apply(km, sm[(0 * L + kz2) * Z::W + w2], sm[(1 * L + kz2) * Z::W + w2],
+                       sm[(2 * L + kz2) * Z::W + w2]);
+        }
+    }
     __syncthreads();
 #pragma unroll
-    for (int c = 0; c < 3; c++) {
-        float2 v[C::E];
+    for (int gg = 0; gg < Z::G; gg++)
 #pragma unroll
-        for (int q = 0; q < C::E; q++) {
-            const int i = t + C::T * q;
-            v[q] = (ok && i < g.n0) ? base[c * cstride + i * istride] : float2{0.0f, 0.0f};
-        }
-        col_forward<L>(v, u[c], w, t, sbuf, stw);
-    }
+        for (int k2 = 0; k2 < Z::T; k2++)
+            u[gg][k2] = sm[(c * L + t * Z::G + gg + Z::E * k2) * Z::W + w];
+    __syncthreads();
+    col_inverse<L, Z::W, Z::S>(u, v, w, t, sbuf, stw);
     if (ok) {
 #pragma unroll
-        for (int gg = 0; gg < C::G; gg++)
-#pragma unroll
-            for (int k2 = 0; k2 < C::T; k2++) {
-                const Kern6f k = kern6_at(g, K, t * C::G + gg + C::E * k2, ky, kx0 + w);
-                kern6_apply(k, u[0][gg][k2], u[1][gg][k2], u[2][gg][k2]);
-            }
-    }
-#pragma unroll
-    for (int c = 0; c < 3; c++) {
-        float2 v[C::E];
-        col_inverse<L>(u[c], v, w, t, sbuf, stw);
-        if (ok) {
-#pragma unroll
-            for (int q = 0; q < C::E; q++) {
-                const int i = t + C::T * q;
-                if (i < g.n0) base[c * cstride + i * istride] = v[q];
-            }
+        for (int q = 0; q < Z::E; q++) {
+            const int i = t + Z::T * q;
+            if (i < g.n0) base[i * istride] = v[q];
         }
     }
 }
@@ -257,21 +274,21 @@ y2d_fast(DemagGeom g, int ntx, const float2 *__restrict__ tw, const float *__res
             const int j = t + C::T * q;
             v[q] = (ok && j < g.n1) ? base[c * cstride + (long)j * g.sx] : float2{0.0f, 0.0f};
         }
-        col_forward<L>(v, u[c], w, t, sbuf, stw);
+        col_forward<L, C::W, C::S>(v, u[c], w, t, sbuf, stw);
     }
     if (ok) {
 #pragma unroll
         for (int gg = 0; gg < C::G; gg++)
 #pragma unroll
             for (int k2 = 0; k2 < C::T; k2++) {
-                const Kern6f k = kern6_at(g, K, 0, t * C::G + gg + C::E * k2, kx0 + w);
-                kern6_apply(k, u[0][gg][k2], u[1][gg][k2], u[2][gg][k2]);
+                const Kern6 k = kern_at(g, K, 0, t * C::G + gg + C::E * k2, kx0 + w);
+                kern_apply(k, u[0][gg][k2], u[1][gg][k2], u[2][gg][k2]);
             }
     }
 #pragma unroll
     for (int c = 0; c < 3; c++) {
         float2 v[C::E];
-        col_inverse<L>(u[c], v, w, t, sbuf, stw);
+        col_inverse<L, C::W, C::S>(u[c], v, w, t, sbuf, stw);
         if (ok) {
 #pragma unroll
             for (int q = 0; q < C::E; q++) {
@@ -432,6 +449,117 @@ xf_fast(DemagGeom g, const float2 *__restrict__ tw, const float2 *__restrict__ r
     }
 }
 
+// Two neighbouring cells (k0 even, k0 + 1 < n2) of one row at once, for
+// uniform material parameters (no maps) and no spin-transfer term: the same
+// per-cell arithmetic as cell_epilogue (exchange6 order dim 0, dim 2, dim 1;
+// SumUpdater order), with 8-byte loads and stores of the cell pair.
+__device__ __forceinline__ float2 ld2(const float *p) { return *reinterpret_cast<const float2 *>(p); }
+__device__ __forceinline__ void st2(float *p, float a, float b)
+{
+    *reinterpret_cast<float2 *>(p) = float2{a, b};
+}
+
+__device__ __forceinline__ int nb_lo(int pos, int len, int wrap)
+{
+    int lo = pos - 1;
+    lo = (lo < 0 && wrap) ? len + lo : lo;
+    return max(lo, 0);
+}
+__device__ __forceinline__ int nb_hi(int pos, int len, int wrap)
+{
+    int hi = pos + 1;
+    hi = (hi == len && wrap) ? hi - len : hi;
+    return min(hi, len - 1);
+}
+
+template <int MODE>
+__device__ __forceinline__ void pair_epilogue(const DemagGeom &g, const FieldParams &fp,
+                                              const EpilogueIO &io, long I0, int i, int j, int k0,
+                                              const float hd0[3], const float hd1[3],
+                                              float &errmax)
+{
+    const long N = g.N;
+    if (MODE == EPI_B) {
+#pragma unroll
+        for (int c = 0; c < 3; c++) st2(io.hout + c * N + I0, hd0[c], hd1[c]);
+        return;
+    }
+    const float *__restrict__ m = io.m;
+    float m0[3], m1[3], H0[3], H1[3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        const float2 a = ld2(m + c * N + I0);
+        m0[c] = a.x;
+        m1[c] = a.y;
+    }
+    // neighbour offsets: dim 0 (planes), dim 2 (along the row), dim 1 (rows)
+    const long s0 = (long)g.n1 * g.n2, s1 = g.n2;
+    const long o0l = (long)(nb_lo(i, g.n0, fp.pbc[0]) - i) * s0;
+    const long o0h = (long)(nb_hi(i, g.n0, fp.pbc[0]) - i) * s0;
+    const long o1l = (long)(nb_lo(j, g.n1, fp.pbc[1]) - j) * s1;
+    const long o1h = (long)(nb_hi(j, g.n1, fp.pbc[1]) - j) * s1;
+    const long o2l = nb_lo(k0, g.n2, fp.pbc[2]) - k0;          // left of cell 0
+    const long o2h = nb_hi(k0 + 1, g.n2, fp.pbc[2]) - (k0 + 1); // right of cell 1
+    const float p0 = fp.ex_pre * fp.ex_c[0], p1 = fp.ex_pre * fp.ex_c[1],
+                p2 = fp.ex_pre * fp.ex_c[2];
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        const float *mc = m + c * N + I0;
+        const float2 zl = ld2(mc + o0l), zh = ld2(mc + o0h);
+        const float2 yl = ld2(mc + o1l), yh = ld2(mc + o1h);
+        const float xl = mc[o2l], xh = mc[1 + o2h];
+        float e0 = p0 * ((zl.x - m0[c]) + (zh.x - m0[c]));
+        float e1 = p0 * ((zl.y - m1[c]) + (zh.y - m1[c]));
+        e0 += p2 * ((xl - m0[c]) + (m1[c] - m0[c]));
+        e1 += p2 * ((m0[c] - m1[c]) + (xh - m1[c]));
+        e0 += p1 * ((yl.x - m0[c]) + (yh.x - m0[c]));
+        e1 += p1 * ((yl.y - m1[c]) + (yh.y - m1[c]));
+        H0[c] = hd0[c] + e0 + fp.hext[c];
+        H1[c] = hd1[c] + e1 + fp.hext[c];
+    }
+    if (fp.anis) {
+        const float k = fp.ku2;
+        const float mu0 = m0[0] * fp.u[0] + m0[1] * fp.u[1] + m0[2] * fp.u[2];
+        const float mu1 = m1[0] * fp.u[0] + m1[1] * fp.u[1] + m1[2] * fp.u[2];
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            H0[c] += k * mu0 * fp.u[c];
+            H1[c] += k * mu1 * fp.u[c];
+        }
+    }
+    if (io.hout) {
+#pragma unroll
+        for (int c = 0; c < 3; c++) st2(io.hout + c * N + I0, H0[c], H1[c]);
+    }
+    if (MODE == EPI_HEFF && io.tout == nullptr) return;
+    float t0[3], t1[3];
+    llg_torque(m0, H0, fp.alpha, t0);
+    llg_torque(m1, H1, fp.alpha, t1);
+    if (io.tout) {
+#pragma unroll
+        for (int c = 0; c < 3; c++) st2(io.tout + c * N + I0, t0[c], t1[c]);
+    }
+    if (MODE == EPI_CORR) {
+        const float hh = 0.5f * io.h, mh = -0.5f * io.h;
+        float a0[3], a1[3];
+        float d = 0.0f;
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const float2 p = ld2(io.tau0 + c * N + I0);
+            a0[c] = m0[c] + (hh * t0[c] + mh * p.x);
+            a1[c] = m1[c] + (hh * t1[c] + mh * p.y);
+            d = fmaxf(d, fmaxf(fabsf(t0[c] - p.x), fabsf(t1[c] - p.y)));
+        }
+        errmax = fmaxf(errmax, (d != d) ? __builtin_inff() : d);
+        const float l0 = sqrtf(a0[0] * a0[0] + a0[1] * a0[1] + a0[2] * a0[2]);
+        const float l1 = sqrtf(a1[0] * a1[0] + a1[1] * a1[1] + a1[2] * a1[2]);
+        const float q0 = (l0 != 0.0f) ? 1.0f / l0 : 0.0f;
+        const float q1 = (l1 != 0.0f) ? 1.0f / l1 : 0.0f;
+#pragma unroll
+        for (int c = 0; c < 3; c++) st2(io.mnew + c * N + I0, a0[c] * q0, a1[c] * q1);
+    }
+}
+
 // xi: C2R along x (imaginary parts of DC and Nyquist ignored), keep k < n2,
 // then the fused field / torque / RK epilogue
 template <int LX, int MODE>
@@ -480,21 +608,24 @@ xi_fast(DemagGeom g, const float2 *__restrict__ tw, const float2 *__restrict__ r
     }
 
     float errmax = 0.0f;
+    // this kernel serves uniform parameters without the spin-transfer term
+    // (fast_xi_applies); cell pairs with 8-byte I/O (n2 is even on this path:
+    // p2 = 2 LX is padded from n2 = LX, or periodic with n2 = 2 LX)
     if (ok) {
         const int i = (int)(row / g.n1), j = (int)(row - (long)i * g.n1);
 #pragma unroll
         for (int q = 0; q < C::E; q++) {
+            // keep each cell pair's memory traffic inside its iteration
+            __builtin_amdgcn_sched_barrier(0);
+            const int k0 = 2 * (t + C::T * q);
+            if (k0 >= g.n2) continue;
+            float hd0[3], hd1[3];
 #pragma unroll
-            for (int s = 0; s < 2; s++) {
-                const int k = 2 * (t + C::T * q) + s;
-                if (k < g.n2) {
-                    const long I = row * g.n2 + k;
-                    float hd[3];
-#pragma unroll
-                    for (int c = 0; c < 3; c++) hd[c] = fp.demag_scale * (s ? v[c][q].y : v[c][q].x);
-                    cell_epilogue<MODE>(g, fp, io, I, i, j, k, hd, errmax);
-                }
+            for (int c = 0; c < 3; c++) {
+                hd0[c] = fp.demag_scale * v[c][q].x;
+                hd1[c] = fp.demag_scale * v[c][q].y;
             }
+            pair_epilogue<MODE>(g, fp, io, row * g.n2 + k0, i, j, k0, hd0, hd1, errmax);
         }
     }
     if (MODE == EPI_CORR) block_max_to(errmax, io.err);
@@ -559,10 +690,10 @@ int fast_z(const DemagPlanImpl *P, hipStream_t st)
     switch (g.p0) {
 #define CASE(L)                                                                              \
     case L: {                                                                                \
-        using C = ColTile<L>;                                                                \
-        const int ntx = div_up(g.h2, C::W);                                                  \
-        allow_lds_fast(z_fast<L>, C::LDS);                                                   \
-        z_fast<L><<<dim3(ntx * g.p1), dim3(C::NT), C::LDS, st>>>(g, ntx, P->fz.tw, P->kern,  \
+        using Z = ZTile<L>;                                                                  \
+        const int ntx = div_up(g.h2, Z::W);                                                  \
+        allow_lds_fast(z_fast<L>, Z::LDS);                                                   \
+        z_fast<L><<<dim3(ntx * g.p1), dim3(Z::NT), Z::LDS, st>>>(g, ntx, P->fz.tw, P->kern,  \
                                                                  P->ybuf);                   \
         break;                                                                               \
     }
@@ -633,6 +764,11 @@ static void launch_xi(const DemagPlanImpl *P, const FieldParams &fp, const Epilo
     allow_lds_fast(xi_fast<L, MODE>, C::LDS);
     xi_fast<L, MODE><<<dim3(nb), dim3(C::NT), C::LDS, st>>>(P->g, P->fx.tw, P->fx.rtw, P->xbuf,
                                                             fp, io);
+}
+
+bool fast_xi_applies(const FieldParams &fp)
+{
+    return !fp.msat_map && !fp.aex_map && !fp.alpha_map && !fp.ku_map && !fp.zl;
 }
 
 int fast_xi(const DemagPlanImpl *P, int mode, const FieldParams &fp, const EpilogueIO &io,

@@ -112,6 +112,8 @@ int fast_yi(const DemagPlanImpl *P, hipStream_t st);
 int fast_y2d(const DemagPlanImpl *P, hipStream_t st);
 int fast_xi(const DemagPlanImpl *P, int mode, const FieldParams &fp, const EpilogueIO &io,
             hipStream_t st);
+// the fast C2R pass covers uniform parameters without the spin-transfer term
+bool fast_xi_applies(const FieldParams &fp);
 
 } // namespace mmx
 

@@ -165,6 +165,12 @@ __global__ void __launch_bounds__(kBlock) axis_fft_f64(double2 *__restrict__ dat
     for (int t = threadIdx.x; t < L; t += blockDim.x) base[(long)t * inner] = R[t];
 }
 
+__global__ void __launch_bounds__(kBlock) zero_strided(float *__restrict__ out, long n)
+{
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x)
+        out[6 * t] = 0.0f;
+}
+
 // Octant extraction with the 1/Np scale; realness statistics over it.
 __global__ void __launch_bounds__(kBlock) extract_octant(const double2 *__restrict__ full, int p0,
                                                          int p1, int p2, int ko0, int ko1,
@@ -180,7 +186,7 @@ __global__ void __launch_bounds__(kBlock) extract_octant(const double2 *__restri
         const int ky = (int)((t / h2) % ko1);
         const int kz = (int)(t / ((long)h2 * ko1));
         const double2 v = full[((long)kz * p1 + ky) * p2 + kx];
-        out[t] = (float)(v.x * inv);
+        out[6 * t] = (float)(v.x * inv); // component slot of `out` set by the caller
         bad = fmaxf(bad, (float)fabs(v.y));
         good = fmaxf(good, (float)fabs(v.x));
     }
@@ -312,9 +318,12 @@ int build_demag_kernel(DemagPlanImpl *P, bool keep_real)
         const long octant = (long)g.ko0 * g.ko1 * g.h2;
         const double inv = 1.0 / (double)Pn;
         for (int c = 0; c < 6; c++) {
-            float *dst = P->kern + c * octant;
+            // interleaved storage: component c of point o at P->kern[6*o + c]
+            // (6*octant floats); extract_octant / zero_strided write out[6*t]
+            float *dst = P->kern + c;
             if (kmax == 0.0f || cmax[c] / kmax < 1e-5f) {
-                if (hipMemset(dst, 0, sizeof(float) * octant) != hipSuccess) { fail(MMX_ERR_HIP); break; }
+                zero_strided<<<dim3(1024), dim3(kBlock)>>>(dst, octant);
+                if (hipGetLastError() != hipSuccess) { set_error("kernel zero launch failed"); fail(MMX_ERR_HIP); break; }
                 continue;
             }
             real_to_complex_f64<<<dim3(4096), dim3(kBlock)>>>(K6 + c * Pn, Pn, full);

@@ -252,8 +252,11 @@ int mmx_demag_plan_kernel(const mmx_demag_plan *plan, float *host)
 {
     MMX_REQUIRE(plan && host, MMX_ERR_ARG, "mmx_demag_plan_kernel: NULL argument");
     const DemagGeom &g = plan->impl.g;
-    MMX_HIP(hipMemcpy(host, plan->impl.kern, sizeof(float) * 6 * (size_t)g.ko0 * g.ko1 * g.h2,
-                      hipMemcpyDeviceToHost));
+    const size_t n = (size_t)g.ko0 * g.ko1 * g.h2;
+    std::vector<float> aos(6 * n);
+    MMX_HIP(hipMemcpy(aos.data(), plan->impl.kern, sizeof(float) * 6 * n, hipMemcpyDeviceToHost));
+    for (size_t o = 0; o < n; o++)
+        for (int c = 0; c < 6; c++) host[c * n + o] = aos[6 * o + c];
     return MMX_OK;
 }
 

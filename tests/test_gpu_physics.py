@@ -1,0 +1,138 @@
+"""Standard problems through the user-order API on the GPU, against the
+reference's own data fixtures (tests/golden/).
+
+* SP4 (tests/stdproblem4.py, fixture tests/stdsolution4.txt): demag +
+  exchange + LLG + adaptive RK12 on the 128x32x1 film geometry.
+* SP5 (examples/stdproblem5.py, fixture certification/stdproblem5/m.txt):
+  the Zhang-Li path.
+
+The tolerances are the physics-level agreement the reference's own fixtures
+allow (see the test docstrings); they are not the 1e-5 field parity, which
+tests/test_gpu_parity.py checks against the oracle.
+"""
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def _say(*a):
+    print("[case]", *a, flush=True)
+
+
+@pytest.fixture(scope="module")
+def api():
+    import torch
+    assert torch.cuda.is_available()
+    import mmx
+    from mmx import api as A
+    mmx.check(mmx.lib().mmx_init(0))
+    return A
+
+
+def test_sp4_against_standard_solution(api):
+    """tests/stdproblem4.py: relax (1,1,0) at alpha = 1 for 2 ns, then
+    B = (-24.6, 4.3, 0) mT at alpha = 0.02; <m>(t) every 10 ps for 1 ns against
+    tests/stdsolution4.txt.  Published SP4 solutions agree with each other to a
+    few percent until the magnetisation reverses (~0.15-0.2 ns) and drift apart
+    afterwards, so the check is tight early and loose late."""
+    _say("SP4")
+    ref = np.loadtxt(os.path.join(GOLDEN, "stdsolution4.txt"))
+    e = api.reset()
+    e.setgridsize(128, 32, 1)
+    e.setcellsize(500e-9 / 128, 125e-9 / 32, 3e-9)
+    e.load("micromagnetism")
+    e.setv("Msat", 800e3)
+    e.setv("Aex", 1.3e-11)
+    e.setv("alpha", 1)
+    e.setv("dt", 1e-15)
+    e.setv("m_maxerror", 1.0 / 3000)
+    e.setarray("m", [[[[1]]], [[[1]]], [[[0]]]])
+    e.run(2e-9)
+    m0 = e.getv("<m>")
+    _say("SP4 relaxed <m>", m0)
+    assert abs(m0[0] - ref[0, 1]) < 0.01 and abs(m0[1] - ref[0, 2]) < 0.01
+    e.setv("B_ext", [-24.6e-3, 4.3e-3, 0.0])
+    e.setv("alpha", 0.02)
+    e.setv("dt", 1e-15)
+    e.setv("t", 0.0)
+    tab = e.autotabulate(["t", "<m>"], period=10e-12)
+    e.run(1e-9)
+    rows = np.array(tab.rows)
+    dev = []
+    for t, mx, my, mz in rows:
+        want = [np.interp(t * 1e9, ref[:, 0], ref[:, c]) for c in (1, 2, 3)]
+        dev.append((t, max(abs(mx - want[0]), abs(my - want[1]), abs(mz - want[2]))))
+    dev = np.array(dev)
+    early = dev[dev[:, 0] <= 0.12e-9, 1].max()
+    late = dev[:, 1].max()
+    _say("SP4 max |d<m>| t<=0.12ns %.4f, all %.4f" % (early, late))
+    assert early < 0.03
+    assert late < 0.35
+
+
+def test_sp5_against_reference_trace(api):
+    """examples/stdproblem5.py: vortex relaxed at alpha = 1 (3 ns: the
+    reference trace starts from a fully centred vortex, which the script's
+    maxtorque criterion does not guarantee), then Zhang-Li STT, j = 1e12 A/m^2
+    along x, alpha = 0.1, xi = 0.05, P = 1 for 10 ns; <m>(t) every 50 ps
+    against certification/stdproblem5/m.txt (mumax2's own output; its relaxed
+    core <mz> is 1.2 % below this method's, see the oracle's SP5 test)."""
+    _say("SP5")
+    ref = np.loadtxt(os.path.join(GOLDEN, "sp5_m.txt"), comments="#")
+    e = api.reset()
+    nx, ny, nz = 32, 32, 1
+    csx, csy, csz = 100e-9 / nx, 100e-9 / ny, 10e-9 / nz
+    e.setgridsize(nx, ny, nz)
+    e.setcellsize(csx, csy, csz)
+    e.load("micromagnetism")
+    e.setv("Msat", 800e3)
+    e.setv("Aex", 1.3e-11)
+    e.setv("alpha", 1.0)
+    e.setv("gamma", 2.211e5)
+    e.setv("dt", 1e-15)
+    e.setv("maxdt", 1e-12)
+    e.setv("m_maxerror", 1e-4)
+    mv = np.zeros((3, nx, ny, nz))
+    for a in range(nx):
+        for b in range(ny):
+            xx = a * csx - 50e-9
+            yy = 50e-9 - b * csy
+            mv[0, a, b, 0] = yy
+            mv[1, a, b, 0] = xx
+            mv[2, a, b, 0] = 40e-9
+    e.setarray("m", mv)
+    e.run(3e-9)
+    e.load("zhang-li")
+    e.setv("alpha", 0.1)
+    e.setv("dt", 1e-15)
+    e.setv("t", 0)
+    e.setv("xi", 0.05)
+    e.setv("polarisation", 1.0)
+    e.setv("j", [1e12, 0, 0])
+    m0 = e.getv("<m>")
+    _say("SP5 relaxed <m>", m0)
+    assert abs(m0[0]) < 1e-3 and abs(m0[1]) < 1e-3 and abs(m0[2] - ref[0, 3]) < 1e-3
+    tab = e.autotabulate(["t", "<m>"], period=50e-12)
+    e.run(10e-9)
+    rows = np.array(tab.rows)
+    dxy = []
+    for t, mx, my, mz in rows:
+        i = np.argmin(np.abs(ref[:, 0] - t))
+        if abs(ref[i, 0] - t) > 2e-12:
+            continue
+        dxy.append((t, max(abs(mx - ref[i, 1]), abs(my - ref[i, 2])), abs(mz - ref[i, 3])))
+    dxy = np.array(dxy)
+    _say("SP5 rows %d, max |d<mx,my>| t<=0.3ns %.4f, all %.4f, max |d<mz>| %.5f" % (
+        len(dxy), dxy[dxy[:, 0] <= 0.3e-9, 1].max(), dxy[:, 1].max(), dxy[:, 2].max()))
+    assert len(dxy) >= 150
+    assert dxy[dxy[:, 0] <= 0.3e-9, 1].max() < 1.5e-2
+    # the core settles where the spin-transfer force balances the confinement:
+    # final <m> within 10 % of the reference's displacement
+    fin = rows[-1]
+    assert abs(fin[1] - ref[-1, 1]) < 0.02 and abs(fin[2] - ref[-1, 2]) < 0.02
+    assert dxy[:, 2].max() < 1e-3
