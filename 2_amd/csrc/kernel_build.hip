@@ -131,6 +131,190 @@ __global__ void __launch_bounds__(kBlock) arne_kernel(ArneArgs a, float *__restr
     }
 }
 
+// ------------------------------------------------------------ Newell, FP64
+//
+// No reference counterpart (the reference's dipole kernel is Kernel_Arne).
+// Newell, Williams & Dunlop, J. Geophys. Res. 98 (1993) 9551: the cell-pair
+// averaged tensor is a 27-point sixth-order difference of f (diagonal) and g
+// (off-diagonal).  That difference cancels ~log10(64 |R|^6 / V^2) digits, so
+// from NEWELL_RSWITCH largest cell sizes on the tensor is the point-dipole
+// tensor averaged over the offset of two points drawn uniformly from the two
+// cells (triangular density 1 - |t| per axis), with a 5-point Gauss rule of
+// that density per axis (error below 1e-10 relative there).  K = -N.
+
+constexpr double kNewellRSwitch = 8.0;
+constexpr int kNewellNQ = 5;
+
+struct NewellArgs {
+    double d[3];
+    double rs2;        // (kNewellRSwitch * largest cell size)^2
+    double t[kNewellNQ], w[kNewellNQ];
+    int e[3];          // octant extent per axis
+};
+
+__device__ inline double nw_f(double x, double y, double z)
+{
+    x = fabs(x); y = fabs(y); z = fabs(z);
+    const double x2 = x * x, y2 = y * y, z2 = z * z;
+    const double R = sqrt(x2 + y2 + z2);
+    double t = 0.0;
+    if (y > 0 && x2 + z2 > 0) t += 0.5 * y * (z2 - x2) * asinh(y / sqrt(x2 + z2));
+    if (z > 0 && x2 + y2 > 0) t += 0.5 * z * (y2 - x2) * asinh(z / sqrt(x2 + y2));
+    if (x * y * z > 0) t -= x * y * z * atan(y * z / (x * R));
+    t += (2.0 * x2 - y2 - z2) * R / 6.0;
+    return t;
+}
+
+__device__ inline double nw_g(double x, double y, double z)
+{
+    const double sgn = ((x < 0) != (y < 0)) ? -1.0 : 1.0;
+    x = fabs(x); y = fabs(y); z = fabs(z);
+    const double x2 = x * x, y2 = y * y, z2 = z * z;
+    const double R = sqrt(x2 + y2 + z2);
+    double t = 0.0;
+    if (z > 0 && x2 + y2 > 0) t += x * y * z * asinh(z / sqrt(x2 + y2));
+    if (x > 0 && y2 + z2 > 0) t += y / 6.0 * (3.0 * z2 - y2) * asinh(x / sqrt(y2 + z2));
+    if (y > 0 && x2 + z2 > 0) t += x / 6.0 * (3.0 * z2 - x2) * asinh(y / sqrt(x2 + z2));
+    if (z > 0) t -= z2 * z / 6.0 * atan(x * y / (z * R));
+    if (y > 0) t -= z * y2 / 2.0 * atan(x * z / (y * R));
+    if (x > 0) t -= z * x2 / 2.0 * atan(y * z / (x * R));
+    t -= x * y * R / 3.0;
+    return sgn * t;
+}
+
+// stored component c = axis pair (a, b), XX,YY,ZZ,YZ,XZ,XY
+__device__ __constant__ int kNwA[6] = {0, 1, 2, 1, 0, 0};
+__device__ __constant__ int kNwB[6] = {0, 1, 2, 2, 2, 1};
+
+// One thread per octant displacement R = (i,j,k) * d, i < e0, j < e1, k < e2;
+// writes the 6 components of K(R) to oct[6*t + c].
+__global__ void __launch_bounds__(kBlock) newell_octant(NewellArgs a, double *__restrict__ oct)
+{
+    const long ne = (long)a.e[0] * a.e[1] * a.e[2];
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ne) return;
+    const int k = (int)(t % a.e[2]);
+    const int j = (int)((t / a.e[2]) % a.e[1]);
+    const int i = (int)(t / ((long)a.e[2] * a.e[1]));
+    const double R[3] = {i * a.d[0], j * a.d[1], k * a.d[2]};
+    const double V = a.d[0] * a.d[1] * a.d[2];
+    double out[6];
+    if (R[0] * R[0] + R[1] * R[1] + R[2] * R[2] >= a.rs2) {
+        double acc[6] = {0, 0, 0, 0, 0, 0};
+        for (int qi = 0; qi < kNewellNQ; qi++)
+            for (int qj = 0; qj < kNewellNQ; qj++)
+                for (int qk = 0; qk < kNewellNQ; qk++) {
+                    const double x[3] = {R[0] + a.t[qi] * a.d[0], R[1] + a.t[qj] * a.d[1],
+                                         R[2] + a.t[qk] * a.d[2]};
+                    const double r2 = x[0] * x[0] + x[1] * x[1] + x[2] * x[2];
+                    const double W = a.w[qi] * a.w[qj] * a.w[qk] / (r2 * r2 * sqrt(r2));
+                    acc[0] += W * (3.0 * x[0] * x[0] - r2);
+                    acc[1] += W * (3.0 * x[1] * x[1] - r2);
+                    acc[2] += W * (3.0 * x[2] * x[2] - r2);
+                    acc[3] += W * (3.0 * x[1] * x[2]);
+                    acc[4] += W * (3.0 * x[0] * x[2]);
+                    acc[5] += W * (3.0 * x[0] * x[1]);
+                }
+        for (int c = 0; c < 6; c++) out[c] = V / (4.0 * kPi) * acc[c];
+    } else {
+        const double w3[3] = {-1.0, 2.0, -1.0};
+        for (int c = 0; c < 6; c++) {
+            const int ax = kNwA[c], bx = kNwB[c];
+            int b2 = bx, c2 = 3 - ax - bx;
+            if (ax == bx) { b2 = (ax + 1) % 3; c2 = (ax + 2) % 3; }
+            double s = 0.0;
+            for (int u = 0; u < 3; u++)
+                for (int v = 0; v < 3; v++)
+                    for (int w = 0; w < 3; w++) {
+                        const double xa = R[ax] + (u - 1) * a.d[ax];
+                        const double xb = R[b2] + (v - 1) * a.d[b2];
+                        const double xc = R[c2] + (w - 1) * a.d[c2];
+                        const double fv = (ax == bx) ? nw_f(xa, xb, xc) : nw_g(xa, xb, xc);
+                        s += w3[u] * w3[v] * w3[w] * fv;
+                    }
+            out[c] = -s / (4.0 * kPi * V);
+        }
+    }
+    // K_ab is odd in R_a and in R_b: exactly 0 on those planes
+    if (i == 0 || k == 0) out[4] = 0.0;          // XZ: axes 0, 2
+    if (j == 0 || k == 0) out[3] = 0.0;          // YZ: axes 1, 2
+    if (i == 0 || j == 0) out[5] = 0.0;          // XY: axes 0, 1
+    for (int c = 0; c < 6; c++) oct[6 * t + c] = out[c];
+}
+
+// Component c of the padded real-space kernel from the octant: displacement
+// of padded index q is q (q < p/2) or q - p (q > p/2); the p/2 planes are 0
+// (the displacement range Kernel_Arne fills).  Off-diagonal components are
+// odd in both of their axes.  Optionally also the float32 copy (real_f32).
+__global__ void __launch_bounds__(kBlock) newell_expand(const double *__restrict__ oct, int e0, int e1,
+                                                        int e2, int p0, int p1, int p2, int c,
+                                                        double2 *__restrict__ full,
+                                                        float *__restrict__ real_f32)
+{
+    const long P = (long)p0 * p1 * p2;
+    const int ax = kNwA[c], bx = kNwB[c];
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < P; t += (long)gridDim.x * blockDim.x) {
+        const int q[3] = {(int)(t / ((long)p1 * p2)), (int)((t / p2) % p1), (int)(t % p2)};
+        const int p[3] = {p0, p1, p2};
+        int disp[3];
+        bool zero = false;
+        for (int x = 0; x < 3; x++) {
+            if (p[x] == 1) { disp[x] = 0; continue; }
+            if (2 * q[x] == p[x]) zero = true;
+            disp[x] = (2 * q[x] < p[x]) ? q[x] : q[x] - p[x];
+        }
+        double v = 0.0;
+        if (!zero) {
+            const long o = ((long)abs(disp[0]) * e1 + abs(disp[1])) * e2 + abs(disp[2]);
+            v = oct[6 * o + c];
+            if (ax != bx) {
+                if (disp[ax] < 0) v = -v;
+                if (disp[bx] < 0) v = -v;
+            }
+        }
+        full[t] = double2{v, 0.0};
+        if (real_f32) real_f32[t] = (float)v;
+    }
+}
+
+// max |oct[6 t + c]| per component c into dst[c] (float bits)
+__global__ void __launch_bounds__(kBlock) maxabs_oct6(const double *__restrict__ oct, long ne,
+                                                      unsigned int *dst)
+{
+    float m[6] = {0, 0, 0, 0, 0, 0};
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < ne; t += (long)gridDim.x * blockDim.x)
+        for (int c = 0; c < 6; c++) m[c] = fmaxf(m[c], (float)fabs(oct[6 * t + c]));
+    for (int c = 0; c < 6; c++) {
+        block_max_to(m[c], dst + c);
+        __syncthreads();
+    }
+}
+
+// 5-point Gauss rule on [-1,1] for the density 1 - |t|: nodes 0, +-sqrt(u1),
+// +-sqrt(u2) with u1, u2 the roots of the quadratic orthogonal to 1 and u
+// under u dmu(u), mu the distribution of u = t^2 (moments E[t^2j] =
+// 2/((2j+1)(2j+2))); weights from the moment equations.
+static void newell_rule5(double t[5], double w[5])
+{
+    long double mu[5];
+    for (int j = 0; j < 5; j++) mu[j] = 2.0L / ((2.0L * j + 1.0L) * (2.0L * j + 2.0L));
+    // u^2 + c1 u + c0:  mu3 + c1 mu2 + c0 mu1 = 0,  mu4 + c1 mu3 + c0 mu2 = 0
+    const long double det = mu[2] * mu[2] - mu[1] * mu[3];
+    const long double c1 = (-mu[3] * mu[2] + mu[1] * mu[4]) / det;
+    const long double c0 = (-mu[2] * mu[4] + mu[3] * mu[3]) / det;
+    const long double disc = sqrtl(c1 * c1 - 4.0L * c0);
+    const long double u1 = (-c1 + disc) / 2.0L, u2 = (-c1 - disc) / 2.0L;
+    // 2 (W1 u1 + W2 u2) = mu1,  2 (W1 u1^2 + W2 u2^2) = mu2
+    const long double A = (mu[2] / 2.0L - u2 * mu[1] / 2.0L) / (u1 - u2);
+    const long double B = mu[1] / 2.0L - A;
+    const long double W1 = A / u1, W2 = B / u2;
+    t[0] = (double)-sqrtl(u1); w[0] = (double)W1;
+    t[1] = (double)-sqrtl(u2); w[1] = (double)W2;
+    t[2] = 0.0;                w[2] = (double)(1.0L - 2.0L * (W1 + W2));
+    t[3] = (double)sqrtl(u2);  w[3] = (double)W2;
+    t[4] = (double)sqrtl(u1);  w[4] = (double)W1;
+}
+
 // max |x| of a float array into *dst (bits of a non-negative float)
 __global__ void __launch_bounds__(kBlock) maxabs_kernel(const float *__restrict__ a, long n,
                                                         unsigned int *dst)
@@ -215,17 +399,21 @@ int build_demag_kernel(DemagPlanImpl *P, bool keep_real)
     const DemagGeom &g = P->g;
     const int p[3] = {g.p0, g.p1, g.p2};
     const long Pn = (long)p[0] * p[1] * p[2];
+    const bool newell = P->cell_kernel_type == MMX_KERNEL_NEWELL;
+    MMX_REQUIRE(P->cell_kernel_type == MMX_KERNEL_ARNE || newell, MMX_ERR_ARG,
+                "kernel type %d not available", P->cell_kernel_type);
     MMX_REQUIRE(p[0] > 0 && p[1] > 1 && p[2] > 1, MMX_ERR_SIZE,
-                "Kernel_Arne needs padded sizes p1, p2 > 1 (got %d %d %d)", p[0], p[1], p[2]);
+                "demag kernel needs padded sizes p1, p2 > 1 (got %d %d %d)", p[0], p[1], p[2]);
     MMX_REQUIRE(p[1] % 2 == 0 && p[2] % 2 == 0 && (p[0] == 1 || p[0] % 2 == 0), MMX_ERR_SIZE,
-                "Kernel_Arne needs even padded sizes (got %d %d %d)", p[0], p[1], p[2]);
+                "demag kernel needs even padded sizes (got %d %d %d)", p[0], p[1], p[2]);
     MMX_REQUIRE(p[0] <= 4096 && p[1] <= 4096 && p[2] <= 4096, MMX_ERR_SIZE,
                 "kernel build: padded sizes above 4096 are not supported (%d %d %d)", p[0], p[1],
                 p[2]);
-    MMX_REQUIRE(P->cell_kernel_type == MMX_KERNEL_ARNE, MMX_ERR_ARG,
-                "kernel type %d not available", P->cell_kernel_type);
+    MMX_REQUIRE(!newell || (P->pbc[0] == 0 && P->pbc[1] == 0 && P->pbc[2] == 0), MMX_ERR_ARG,
+                "the Newell kernel supports non-periodic grids only");
 
-    float *K6 = nullptr;
+    float *K6 = nullptr;     // Arne: float32 real-space kernel (6 x Pn); Newell: kept copy
+    double *oct = nullptr;   // Newell: FP64 octant table (6 per displacement)
     double2 *full = nullptr;
     unsigned int *stats = nullptr;
     void *twd = nullptr;
@@ -234,56 +422,73 @@ int build_demag_kernel(DemagPlanImpl *P, bool keep_real)
         rc = code;
         return code;
     };
-    hipError_t e = hipMalloc(&K6, sizeof(float) * 6 * Pn);
-    if (e != hipSuccess) {
-        set_error("kernel build: cannot allocate %ld bytes", (long)(sizeof(float) * 6 * Pn));
-        return MMX_ERR_NOMEM;
-    }
-    e = hipMalloc(&full, sizeof(double2) * Pn);
-    if (e == hipSuccess) e = hipMalloc(&stats, sizeof(unsigned int) * 16);
-    if (e != hipSuccess) {
+    int e[3];
+    for (int a = 0; a < 3; a++) e[a] = (p[a] == 1) ? 1 : p[a] / 2;
+    const long ne = (long)e[0] * e[1] * e[2];
+    hipError_t err = hipSuccess;
+    if (!newell || keep_real) err = hipMalloc(&K6, sizeof(float) * 6 * Pn);
+    if (err == hipSuccess && newell) err = hipMalloc(&oct, sizeof(double) * 6 * ne);
+    if (err == hipSuccess) err = hipMalloc(&full, sizeof(double2) * Pn);
+    if (err == hipSuccess) err = hipMalloc(&stats, sizeof(unsigned int) * 16);
+    if (err != hipSuccess) {
         (void)hipFree(K6);
+        (void)hipFree(oct);
         (void)hipFree(full);
-        set_error("kernel build: cannot allocate the FP64 work array");
+        (void)hipFree(stats);
+        set_error("kernel build: cannot allocate the FP64 work arrays (%ld points)", Pn);
         return MMX_ERR_NOMEM;
     }
     do {
-        if (hipMemset(K6, 0, sizeof(float) * 6 * Pn) != hipSuccess) { fail(MMX_ERR_HIP); break; }
+        if (K6 && hipMemset(K6, 0, sizeof(float) * 6 * Pn) != hipSuccess) { fail(MMX_ERR_HIP); break; }
         if (hipMemset(stats, 0, sizeof(unsigned int) * 16) != hipSuccess) { fail(MMX_ERR_HIP); break; }
 
-        // ranges of Kernel_Arne (kernel_magnetostatic.go:50-65)
-        ArneArgs a{};
-        for (int i = 0; i < 3; i++) {
-            a.p[i] = p[i];
-            a.cell[i] = P->cell[i];
-            a.pbc[i] = P->pbc[i];
-        }
-        a.x1 = -(p[0] - 1) / 2; a.x2 = p[0] / 2 - 1;
-        a.y1 = -(p[1] - 1) / 2; a.y2 = p[1] / 2 - 1;
-        a.z1 = -(p[2] - 1) / 2; a.z2 = p[2] / 2 - 1;
-        if (p[0] == 1 && P->pbc[0] == 0) a.x2 = 0;
-        a.x1 *= (P->pbc[0] + 1); a.x2 *= (P->pbc[0] + 1);
-        a.y1 *= (P->pbc[1] + 1); a.y2 *= (P->pbc[1] + 1);
-        a.z1 *= (P->pbc[2] + 1); a.z2 *= (P->pbc[2] + 1);
-        double L = P->cell[0];
-        if (P->cell[1] < L) L = P->cell[1];
-        if (P->cell[2] < L) L = P->cell[2];
-        a.L = L;
-        a.accuracy = (double)P->accuracy;
-        a.accumulate = (P->pbc[0] || P->pbc[1] || P->pbc[2]) ? 1 : 0;
-        const long npts = (long)(a.x2 - a.x1 + 1) * (a.y2 - a.y1 + 1) * (a.z2 - a.z1 + 1);
-        for (int s = 0; s < 3; s++) {
-            a.s = s;
-            arne_kernel<<<dim3(div_up(npts, kBlock)), dim3(kBlock)>>>(a, K6);
-            if (hipGetLastError() != hipSuccess) { set_error("arne_kernel launch failed"); fail(MMX_ERR_HIP); break; }
-        }
-        if (rc) break;
-
-        // IsZero test per component (maxwell_convolution.go:412-418)
         const int gb = 1024;
-        for (int c = 0; c < 6; c++)
-            maxabs_kernel<<<dim3(gb), dim3(kBlock)>>>(K6 + c * Pn, Pn, stats + c);
-        if (hipGetLastError() != hipSuccess) { set_error("maxabs launch failed"); fail(MMX_ERR_HIP); break; }
+        if (newell) {
+            NewellArgs a{};
+            double dm = 0.0;
+            for (int x = 0; x < 3; x++) {
+                a.d[x] = P->cell[x];
+                a.e[x] = e[x];
+                dm = fmax(dm, P->cell[x]);
+            }
+            a.rs2 = kNewellRSwitch * kNewellRSwitch * dm * dm;
+            newell_rule5(a.t, a.w);
+            newell_octant<<<dim3(div_up(ne, kBlock)), dim3(kBlock)>>>(a, oct);
+            maxabs_oct6<<<dim3(gb), dim3(kBlock)>>>(oct, ne, stats);
+            if (hipGetLastError() != hipSuccess) { set_error("newell_octant launch failed"); fail(MMX_ERR_HIP); break; }
+        } else {
+            // ranges of Kernel_Arne (kernel_magnetostatic.go:50-65)
+            ArneArgs a{};
+            for (int i = 0; i < 3; i++) {
+                a.p[i] = p[i];
+                a.cell[i] = P->cell[i];
+                a.pbc[i] = P->pbc[i];
+            }
+            a.x1 = -(p[0] - 1) / 2; a.x2 = p[0] / 2 - 1;
+            a.y1 = -(p[1] - 1) / 2; a.y2 = p[1] / 2 - 1;
+            a.z1 = -(p[2] - 1) / 2; a.z2 = p[2] / 2 - 1;
+            if (p[0] == 1 && P->pbc[0] == 0) a.x2 = 0;
+            a.x1 *= (P->pbc[0] + 1); a.x2 *= (P->pbc[0] + 1);
+            a.y1 *= (P->pbc[1] + 1); a.y2 *= (P->pbc[1] + 1);
+            a.z1 *= (P->pbc[2] + 1); a.z2 *= (P->pbc[2] + 1);
+            double L = P->cell[0];
+            if (P->cell[1] < L) L = P->cell[1];
+            if (P->cell[2] < L) L = P->cell[2];
+            a.L = L;
+            a.accuracy = (double)P->accuracy;
+            a.accumulate = (P->pbc[0] || P->pbc[1] || P->pbc[2]) ? 1 : 0;
+            const long npts = (long)(a.x2 - a.x1 + 1) * (a.y2 - a.y1 + 1) * (a.z2 - a.z1 + 1);
+            for (int s = 0; s < 3; s++) {
+                a.s = s;
+                arne_kernel<<<dim3(div_up(npts, kBlock)), dim3(kBlock)>>>(a, K6);
+                if (hipGetLastError() != hipSuccess) { set_error("arne_kernel launch failed"); fail(MMX_ERR_HIP); break; }
+            }
+            if (rc) break;
+            // IsZero test per component (maxwell_convolution.go:412-418)
+            for (int c = 0; c < 6; c++)
+                maxabs_kernel<<<dim3(gb), dim3(kBlock)>>>(K6 + c * Pn, Pn, stats + c);
+            if (hipGetLastError() != hipSuccess) { set_error("maxabs launch failed"); fail(MMX_ERR_HIP); break; }
+        }
         unsigned int hs[16];
         if (hipMemcpy(hs, stats, sizeof(hs), hipMemcpyDeviceToHost) != hipSuccess) { fail(MMX_ERR_HIP); break; }
         float cmax[6], kmax = 0.0f;
@@ -326,7 +531,11 @@ int build_demag_kernel(DemagPlanImpl *P, bool keep_real)
                 if (hipGetLastError() != hipSuccess) { set_error("kernel zero launch failed"); fail(MMX_ERR_HIP); break; }
                 continue;
             }
-            real_to_complex_f64<<<dim3(4096), dim3(kBlock)>>>(K6 + c * Pn, Pn, full);
+            if (newell)
+                newell_expand<<<dim3(4096), dim3(kBlock)>>>(oct, e[0], e[1], e[2], p[0], p[1], p[2], c,
+                                                            full, K6 ? K6 + c * Pn : nullptr);
+            else
+                real_to_complex_f64<<<dim3(4096), dim3(kBlock)>>>(K6 + c * Pn, Pn, full);
             long inner = 1;
             for (int ax = 2; ax >= 0; ax--) {
                 if (p[ax] > 1) {
@@ -356,6 +565,7 @@ int build_demag_kernel(DemagPlanImpl *P, bool keep_real)
         K6 = nullptr;
     }
     (void)hipFree(K6);
+    (void)hipFree(oct);
     (void)hipFree(full);
     (void)hipFree(stats);
     (void)hipFree(twd);

@@ -166,7 +166,10 @@ typedef struct mmx_demag_plan mmx_demag_plan;
 
 /* MaxwellPlan.init + loadDipoleKernel + LoadKernel.  Builds the kernel in
  * FP64 on the GPU, transforms it in FP64, stores 6 real symmetric components
- * (octant) in float32.  Sizes in internal order. */
+ * (octant) in float32.  Sizes in internal order.  MMX_KERNEL_ARNE is the
+ * reference's Kernel_Arne at the given accuracy; MMX_KERNEL_NEWELL is the
+ * analytic cell-averaged tensor (Newell, Williams & Dunlop 1993) in FP64 with
+ * a quadrature far field (accuracy ignored; non-periodic grids only). */
 int mmx_demag_plan_create(mmx_demag_plan **plan, int n0, int n1, int n2,
                           double cell0, double cell1, double cell2, int pbc0,
                           int pbc1, int pbc2, int kernel_type, int accuracy);
@@ -211,12 +214,15 @@ int mmx_sim_destroy(mmx_sim *sim);
  * "demag", "exchange6", "zeeman", "llg", "solver/rk12", "maxtorque" are
  * already loaded). */
 int mmx_sim_load(mmx_sim *sim, const char *module);
-/* Demag kernel choice before the first update (default MMX_KERNEL_ARNE, 6). */
+/* Demag kernel choice (default MMX_KERNEL_ARNE, accuracy 6); a change
+ * rebuilds the plan at the next update. */
 int mmx_sim_set_kernel(mmx_sim *sim, int kernel_type, int accuracy);
 /* setv/getv on VALUE/MASK multipliers: Msat, Aex, alpha, gamma, B_ext[3],
  * Ku, anisU[3], j[3], polarisation, xi, t, dt, mindt, maxdt, m_maxerror;
  * read-only: step, badsteps, m_error, m_peakerror, <m>[3], maxtorque,
- * evals (H_eff evaluations so far). */
+ * evals (H_eff evaluations so far), and the energies of micromag/energy
+ * (modules/mod_energy.go:24-100): E_zeeman, E_ex, E_demag, E_anis and their
+ * sum E, in joule, as FP64 sums with compensated partials. */
 int mmx_sim_setv(mmx_sim *sim, const char *name, const double *v, int ncomp);
 int mmx_sim_getv(mmx_sim *sim, const char *name, double *v, int ncomp);
 /* setmask: spatial maps (host float32, N per component) for Msat, Aex,

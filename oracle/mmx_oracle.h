@@ -44,6 +44,19 @@ void orc_pad_size(const int n[3], const int pbc[3], int p[3]);
 int orc_kernel_arne(const int p[3], const double cell[3], const int pbc[3],
                     int accuracy, float *K9);
 
+/* FP64 Newell tensor (newell.c; no reference counterpart).  K = -N for
+ * displacement R (internal axes) and cell d, stored order XX,YY,ZZ,YZ,XZ,XY.
+ * method 0: analytic formulas below 8 largest cell sizes, 5-point-per-axis
+ * far-field rule beyond (the production method); 1: analytic FP64 everywhere;
+ * 2: analytic in __float128; 3: far-field rule (5 points); 4: far-field rule
+ * (4 points). */
+void orc_newell_point(const double R[3], const double d[3], int method, double out6[6]);
+/* n-point (n = 4, 5) Gauss rule for the density 1 - |t| on [-1, 1]. */
+int orc_newell_rule(int n, double *t, double *w);
+/* Padded real-space Newell kernel, 9 x p0*p1*p2 doubles (orc_kernel_arne's
+ * layout); non-periodic only (-1 otherwise). */
+int orc_kernel_newell(const int p[3], const double cell[3], const int pbc[3], double *K9d);
+
 /* In-place complex FFT of one contiguous line, interleaved (re,im), length n,
  * sign -1 = forward (exp(-2 pi i jk/n)), +1 = backward, unnormalised. */
 int orc_fft_line(double *z, int n, int sign);

@@ -64,6 +64,10 @@ def _load():
     lib.orc_maxnorm.restype = C.c_double
     lib.orc_maxdiff.argtypes = [C.c_long, _dp, _dp]
     lib.orc_maxdiff.restype = C.c_double
+    lib.orc_newell_point.argtypes = [_dp, _dp, C.c_int, _dp]
+    lib.orc_newell_point.restype = None
+    lib.orc_newell_rule.argtypes = [C.c_int, _dp, _dp]
+    lib.orc_kernel_newell.argtypes = [_ip, _dp, _ip, _dp]
     return lib
 
 
@@ -108,6 +112,29 @@ def kernel_arne(p, cell, pbc=(0, 0, 0), accuracy=6):
     if rc:
         raise ValueError("orc_kernel_arne rejected size %s" % (p,))
     return K
+
+
+def kernel_newell(p, cell, pbc=(0, 0, 0)):
+    """FP64 Newell kernel on the padded grid p: float64 array (9, p0, p1, p2)."""
+    K = np.zeros((9,) + tuple(p), dtype=np.float64)
+    rc = lib().orc_kernel_newell(_i3(p), _d3(cell), _i3(pbc), _ptr(K))
+    if rc:
+        raise ValueError("orc_kernel_newell rejected %s pbc %s" % (p, pbc))
+    return K
+
+
+def newell_point(R, d, method=0):
+    """K = -N at displacement R for cell d (6 stored components)."""
+    out = np.zeros(6)
+    lib().orc_newell_point(_d3(R), _d3(d), int(method), _ptr(out))
+    return out
+
+
+def newell_rule(n):
+    t, w = np.zeros(n), np.zeros(n)
+    if lib().orc_newell_rule(int(n), _ptr(t), _ptr(w)):
+        raise ValueError("rule order %d" % n)
+    return t, w
 
 
 def fft_line(z, sign=-1):
@@ -246,6 +273,8 @@ class Problem:
             if kernel is None:
                 K9 = kernel_arne(self.p, self.cell, self.pbc, accuracy)
                 kernel, _ = kernel_fft(self.p, K9=K9)
+            elif isinstance(kernel, str) and kernel == "newell":
+                kernel, _ = kernel_fft(self.p, K9d=kernel_newell(self.p, self.cell, self.pbc))
             self.khat6 = np.ascontiguousarray(kernel, dtype=np.float64)
             P.khat6 = _ptr(self.khat6)
         if ku is not None:
