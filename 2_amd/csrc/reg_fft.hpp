@@ -49,7 +49,7 @@ constexpr double cx_cs(long k, long n, bool want_sin)
     const long quad = num / n;
     const double frac = (double)(num - quad * n) / (double)n; // in [0,1)
     const double hp = 1.57079632679489661923132169163975144;
-    double c, s;
+    double c = 0.0, s = 0.0;
     if (frac <= 0.5) {
         c = cx_cos_small(frac * hp);
         s = cx_sin_small(frac * hp);

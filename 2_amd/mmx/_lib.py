@@ -62,6 +62,14 @@ _SIGS = {
     "mmx_sim_run": (_i, [_vp, _d]),
     "mmx_sim_try_step": (_i, [_vp, _d, C.POINTER(_d)]),
     "mmx_sim_time_passes": (_i, [_vp, _i, C.POINTER(_d), C.POINTER(_i)]),
+    "mmx_batch_create": (_i, [C.POINTER(_vp), C.POINTER(_vp), _i]),
+    "mmx_batch_destroy": (_i, [_vp]),
+    "mmx_batch_run": (_i, [_vp, _d]),
+    "mmx_batch_steps": (_i, [_vp, _i]),
+    "mmx_batch_set_graph": (_i, [_vp, _i]),
+    "mmx_batch_tune": (_i, [_vp, _i, C.POINTER(_i)]),
+    "mmx_batch_set_lds": (_i, [_vp, C.POINTER(_i)]),
+    "mmx_batch_info": (_i, [_vp, C.POINTER(_i), C.POINTER(_i)]),
 }
 
 EXPORTED_SYMBOLS = tuple(sorted(_SIGS))

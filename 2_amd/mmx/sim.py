@@ -114,6 +114,57 @@ class Sim:
         return err.value
 
 
+class Batch:
+    """Many simulations stepped together (mmx_batch): one launch per demag
+    pass for all of them, RK12 step control on the device, tries replayed
+    from a hipGraph.  Borrows the Sim objects (keeps references to them)."""
+
+    PASSES = ("xf", "y", "z", "yi", "xi")
+
+    def __init__(self, sims, tries_per_graph=8):
+        self.sims = list(sims)
+        arr = (C.c_void_p * len(self.sims))(*[s._h.value for s in self.sims])
+        h = C.c_void_p()
+        check(lib().mmx_batch_create(C.byref(h), arr, len(self.sims)))
+        self._h = h
+        self.set_graph(tries_per_graph)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().mmx_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_graph(self, tries_per_graph):
+        check(lib().mmx_batch_set_graph(self._h, int(tries_per_graph)))
+
+    def run(self, duration):
+        check(lib().mmx_batch_run(self._h, float(duration)))
+
+    def steps(self, n):
+        check(lib().mmx_batch_steps(self._h, int(n)))
+
+    def tune(self, reps=3):
+        out = (C.c_int * 5)()
+        check(lib().mmx_batch_tune(self._h, int(reps), out))
+        return dict(zip(self.PASSES, list(out)))
+
+    def set_lds(self, kb):
+        arr = (C.c_int * 5)(*[int(kb[p]) for p in self.PASSES])
+        check(lib().mmx_batch_set_lds(self._h, arr))
+
+    def info(self):
+        nb = (C.c_int * 5)()
+        lds = (C.c_int * 5)()
+        check(lib().mmx_batch_info(self._h, nb, lds))
+        return {p: {"blocks": nb[i], "lds_bytes": lds[i]} for i, p in enumerate(self.PASSES)}
+
+
 class DemagPlan:
     """MaxwellPlan for the demag source (mmx_demag_plan)."""
 

@@ -254,6 +254,38 @@ int mmx_sim_time_passes(mmx_sim *sim, int reps, double *ms, int *npass);
  * control (no reference counterpart; used to compare m(t) step by step). */
 int mmx_sim_try_step(mmx_sim *sim, double dt, double *err);
 
+/* ------------------------------------------------------------------------
+ * Batches: many independent simulations of uneven sizes stepped together
+ * (no reference counterpart; the reference runs one simulation per process,
+ * Engine.Run engine.go:489-524).  One launch per demag pass covers every
+ * simulation, the RK12 controller (solve_rk12.go:124-214) runs on the device
+ * after each try, and tries are replayed from a hipGraph.  The batch borrows
+ * the simulations: they must outlive it, and their settings are re-read at
+ * every run/steps call; afterwards each simulation's state (m, t, dt, step,
+ * ...) is the one its own mmx_sim_run would have produced, up to float32
+ * rounding (the batch uses the any-length FFT passes).
+ * ---------------------------------------------------------------------- */
+typedef struct mmx_batch mmx_batch;
+
+int mmx_batch_create(mmx_batch **batch, mmx_sim *const *sims, int nsim);
+int mmx_batch_destroy(mmx_batch *batch);
+/* Engine.Run / Steps per simulation: each one advances until t >= t_start +
+ * duration (or has taken nsteps accepted steps). */
+int mmx_batch_run(mmx_batch *batch, double duration);
+int mmx_batch_steps(mmx_batch *batch, int nsteps);
+/* Tries per graph replay (default 8); 0 = plain launches, no graph. */
+int mmx_batch_set_graph(mmx_batch *batch, int tries_per_graph);
+/* Autotuner: for each of the 5 passes (x forward, y forward or film y,
+ * z, y inverse, x inverse) times the pass over the batch at LDS budgets of
+ * 8..128 KB per workgroup (every simulation's tile shape follows from the
+ * budget) and keeps the fastest; lds_kb[5] receives the choices (may be
+ * NULL).  Does not change m or the solver state. */
+int mmx_batch_tune(mmx_batch *batch, int reps, int *lds_kb);
+/* Sets the per-pass LDS budgets (KB, 1..160) directly. */
+int mmx_batch_set_lds(mmx_batch *batch, const int *lds_kb);
+/* Workgroups per pass launch and dynamic LDS bytes per workgroup (5 each). */
+int mmx_batch_info(const mmx_batch *batch, int *nblocks, int *lds_bytes);
+
 #ifdef __cplusplus
 }
 #endif

@@ -365,6 +365,49 @@ int orc_demag_kernel_fft(const int p[3], const float *K9, const double *K9d,
     return 0;
 }
 
+/* The same transform for a kernel whose components are generated on demand
+ * (fill writes component c as interleaved complex (re, 0) for all p0*p1*p2
+ * points); cmax6 holds max|K_c| for the IsZero test.  Keeps one complex
+ * work array instead of the 9-component real-space kernel. */
+int orc_demag_kernel_fft_cb(const int p[3], const double cmax6[6],
+                            void (*fill)(void *ctx, int c, double *re_im), void *ctx,
+                            double *khat6, double *bad_over_good)
+{
+    const long P = (long)p[0] * p[1] * p[2];
+    const int h2 = p[2] / 2 + 1;
+    const long H = (long)p[0] * p[1] * h2;
+    double kmax = 0;
+    for (int c = 0; c < 6; c++)
+        if (cmax6[c] > kmax) kmax = cmax6[c];
+    cplx *buf = (cplx *)malloc(sizeof(cplx) * P);
+    if (!buf) return -2;
+    double maxbad = 0, maxgood = 0;
+    const double inv = 1.0 / (double)P;
+    const int dims[3] = {p[0], p[1], p[2]};
+    for (int c = 0; c < 6; c++) {
+        double *dst = khat6 + c * H;
+        if (kmax == 0 || cmax6[c] / kmax < 1e-5) {
+            memset(dst, 0, sizeof(double) * H);
+            continue;
+        }
+        fill(ctx, c, (double *)buf);
+        fft_axis(buf, dims, 2, p[2], p[2], -1);
+        fft_axis(buf, dims, 1, p[1], p[1], -1);
+        fft_axis(buf, dims, 0, p[0], p[0], -1);
+#pragma omp parallel for reduction(max : maxbad, maxgood)
+        for (long ij = 0; ij < (long)p[0] * p[1]; ij++)
+            for (int k = 0; k < h2; k++) {
+                cplx v = buf[ij * p[2] + k];
+                if (fabs(cimag(v)) > maxbad) maxbad = fabs(cimag(v));
+                if (fabs(creal(v)) > maxgood) maxgood = fabs(creal(v));
+                dst[ij * h2 + k] = creal(v) * inv;
+            }
+    }
+    free(buf);
+    if (bad_over_good) *bad_over_good = maxgood > 0 ? maxbad / maxgood : 0;
+    return 0;
+}
+
 /* stored component for (s,d): XX,YY,ZZ,YZ,XZ,XY (upper triangle, the lower
  * one aliases it: maxwell_convolution.go:372-384) */
 static int symm_idx(int s, int d)

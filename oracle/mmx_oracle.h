@@ -56,6 +56,14 @@ int orc_newell_rule(int n, double *t, double *w);
 /* Padded real-space Newell kernel, 9 x p0*p1*p2 doubles (orc_kernel_arne's
  * layout); non-periodic only (-1 otherwise). */
 int orc_kernel_newell(const int p[3], const double cell[3], const int pbc[3], double *K9d);
+/* Its transform (as orc_demag_kernel_fft), without the real-space kernel. */
+int orc_newell_khat(const int p[3], const double cell[3], const int pbc[3], double *khat6,
+                    double *bad_over_good);
+/* orc_demag_kernel_fft for components generated on demand by fill(ctx, c,
+ * interleaved complex out[p0*p1*p2]); cmax6 = max|K_c| for the IsZero test. */
+int orc_demag_kernel_fft_cb(const int p[3], const double cmax6[6],
+                            void (*fill)(void *ctx, int c, double *re_im), void *ctx,
+                            double *khat6, double *bad_over_good);
 
 /* In-place complex FFT of one contiguous line, interleaved (re,im), length n,
  * sign -1 = forward (exp(-2 pi i jk/n)), +1 = backward, unnormalised. */

@@ -241,6 +241,80 @@ void orc_newell_point(const double R[3], const double d[3], int method, double o
         if (R[kPairA[c]] == 0.0 || R[kPairB[c]] == 0.0) out6[c] = 0.0;
 }
 
+/* octant table of K at displacements (i,j,k) * cell, e = octant extent */
+static double *newell_octant_table(const int p[3], const double cell[3], int e[3], long *ne_out)
+{
+    for (int a = 0; a < 3; a++) e[a] = (p[a] == 1) ? 1 : p[a] / 2;
+    const long ne = (long)e[0] * e[1] * e[2];
+    double *oct = (double *)malloc(sizeof(double) * 6 * ne);
+    if (!oct) return NULL;
+#pragma omp parallel for schedule(dynamic, 64)
+    for (long t = 0; t < ne; t++) {
+        const int k = (int)(t % e[2]), j = (int)((t / e[2]) % e[1]), i = (int)(t / ((long)e[2] * e[1]));
+        const double R[3] = {i * cell[0], j * cell[1], k * cell[2]};
+        orc_newell_point(R, cell, 0, oct + 6 * t);
+    }
+    *ne_out = ne;
+    return oct;
+}
+
+/* value of component c at padded index t from the octant table (0 on the
+ * p/2 planes; off-diagonal components odd in both of their axes) */
+static double newell_at(const double *oct, const int p[3], const int e[3], int c, long t)
+{
+    const int z = (int)(t % p[2]), y = (int)((t / p[2]) % p[1]), x = (int)(t / ((long)p[2] * p[1]));
+    const int q[3] = {x, y, z};
+    int disp[3];
+    for (int a = 0; a < 3; a++) {
+        if (p[a] == 1) { disp[a] = 0; continue; }
+        if (2 * q[a] == p[a]) return 0.0;
+        disp[a] = (2 * q[a] < p[a]) ? q[a] : q[a] - p[a];
+    }
+    const long o = ((long)abs(disp[0]) * e[1] + abs(disp[1])) * e[2] + abs(disp[2]);
+    double v = oct[6 * o + c];
+    const int a = kPairA[c], b = kPairB[c];
+    if (a != b) {
+        if (disp[a] < 0) v = -v;
+        if (disp[b] < 0) v = -v;
+    }
+    return v;
+}
+
+struct newell_ctx {
+    const double *oct;
+    int p[3], e[3];
+};
+
+static void newell_fill(void *vctx, int c, double *re_im)
+{
+    const struct newell_ctx *x = (const struct newell_ctx *)vctx;
+    const long P = (long)x->p[0] * x->p[1] * x->p[2];
+#pragma omp parallel for schedule(static)
+    for (long t = 0; t < P; t++) {
+        re_im[2 * t] = newell_at(x->oct, x->p, x->e, c, t);
+        re_im[2 * t + 1] = 0.0;
+    }
+}
+
+int orc_newell_khat(const int p[3], const double cell[3], const int pbc[3], double *khat6,
+                    double *bad_over_good)
+{
+    if (pbc[0] || pbc[1] || pbc[2]) return -1;
+    struct newell_ctx x;
+    long ne = 0;
+    double *oct = newell_octant_table(p, cell, x.e, &ne);
+    if (!oct) return -2;
+    double cmax[6] = {0, 0, 0, 0, 0, 0};
+    for (long t = 0; t < ne; t++)
+        for (int c = 0; c < 6; c++)
+            if (fabs(oct[6 * t + c]) > cmax[c]) cmax[c] = fabs(oct[6 * t + c]);
+    x.oct = oct;
+    for (int a = 0; a < 3; a++) x.p[a] = p[a];
+    const int rc = orc_demag_kernel_fft_cb(p, cmax, newell_fill, &x, khat6, bad_over_good);
+    free(oct);
+    return rc;
+}
+
 /* The padded real-space kernel with the Newell tensor, laid out like
  * orc_kernel_arne's output but in double: 9 x p0*p1*p2, tensor order
  * XX,YY,ZZ,YZ,XZ,XY,ZY,ZX,YX.  Displacements |R_a| <= p_a/2 - 1 (the range
@@ -251,38 +325,12 @@ int orc_kernel_newell(const int p[3], const double cell[3], const int pbc[3], do
     if (pbc[0] || pbc[1] || pbc[2]) return -1;
     const long P = (long)p[0] * p[1] * p[2];
     int e[3];
-    for (int a = 0; a < 3; a++) e[a] = (p[a] == 1) ? 1 : p[a] / 2; /* octant extent */
-    const long ne = (long)e[0] * e[1] * e[2];
-    double *oct = (double *)malloc(sizeof(double) * 6 * ne);
+    long ne = 0;
+    double *oct = newell_octant_table(p, cell, e, &ne);
     if (!oct) return -2;
-#pragma omp parallel for schedule(dynamic, 64)
-    for (long t = 0; t < ne; t++) {
-        const int k = (int)(t % e[2]), j = (int)((t / e[2]) % e[1]), i = (int)(t / ((long)e[2] * e[1]));
-        const double R[3] = {i * cell[0], j * cell[1], k * cell[2]};
-        orc_newell_point(R, cell, 0, oct + 6 * t);
-    }
-    memset(K9d, 0, sizeof(double) * 9 * P);
-#pragma omp parallel for
+#pragma omp parallel for schedule(static)
     for (long t = 0; t < P; t++) {
-        const int z = (int)(t % p[2]), y = (int)((t / p[2]) % p[1]), x = (int)(t / ((long)p[2] * p[1]));
-        const int q[3] = {x, y, z};
-        int disp[3], skip = 0;
-        for (int a = 0; a < 3; a++) {
-            if (p[a] == 1) { disp[a] = 0; continue; }
-            if (q[a] == p[a] / 2) { skip = 1; break; }
-            disp[a] = (q[a] < p[a] / 2) ? q[a] : q[a] - p[a];
-        }
-        if (skip) continue;
-        const long o = ((long)abs(disp[0]) * e[1] + abs(disp[1])) * e[2] + abs(disp[2]);
-        for (int c = 0; c < 6; c++) {
-            const int a = kPairA[c], b = kPairB[c];
-            double v = oct[6 * o + c];
-            if (a != b) {
-                if (disp[a] < 0) v = -v;
-                if (disp[b] < 0) v = -v;
-            }
-            K9d[c * P + t] = v;
-        }
+        for (int c = 0; c < 6; c++) K9d[c * P + t] = newell_at(oct, p, e, c, t);
         K9d[6 * P + t] = K9d[3 * P + t]; /* ZY = YZ */
         K9d[7 * P + t] = K9d[4 * P + t]; /* ZX = XZ */
         K9d[8 * P + t] = K9d[5 * P + t]; /* YX = XY */

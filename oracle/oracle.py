@@ -68,6 +68,7 @@ def _load():
     lib.orc_newell_point.restype = None
     lib.orc_newell_rule.argtypes = [C.c_int, _dp, _dp]
     lib.orc_kernel_newell.argtypes = [_ip, _dp, _ip, _dp]
+    lib.orc_newell_khat.argtypes = [_ip, _dp, _ip, _dp, _dp]
     return lib
 
 
@@ -121,6 +122,17 @@ def kernel_newell(p, cell, pbc=(0, 0, 0)):
     if rc:
         raise ValueError("orc_kernel_newell rejected %s pbc %s" % (p, pbc))
     return K
+
+
+def newell_khat(p, cell, pbc=(0, 0, 0)):
+    """Transformed Newell kernel (as kernel_fft) without the real-space kernel."""
+    h2 = p[2] // 2 + 1
+    kh = np.zeros((6, p[0], p[1], h2))
+    bad = C.c_double(0)
+    rc = lib().orc_newell_khat(_i3(p), _d3(cell), _i3(pbc), _ptr(kh), C.byref(bad))
+    if rc:
+        raise ValueError("orc_newell_khat rejected %s pbc %s" % (p, pbc))
+    return kh, bad.value
 
 
 def newell_point(R, d, method=0):
@@ -274,7 +286,7 @@ class Problem:
                 K9 = kernel_arne(self.p, self.cell, self.pbc, accuracy)
                 kernel, _ = kernel_fft(self.p, K9=K9)
             elif isinstance(kernel, str) and kernel == "newell":
-                kernel, _ = kernel_fft(self.p, K9d=kernel_newell(self.p, self.cell, self.pbc))
+                kernel, _ = newell_khat(self.p, self.cell, self.pbc)
             self.khat6 = np.ascontiguousarray(kernel, dtype=np.float64)
             P.khat6 = _ptr(self.khat6)
         if ku is not None:

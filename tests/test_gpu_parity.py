@@ -83,7 +83,11 @@ CONV_CASES = [((1, 32, 128), (3e-9, 3.90625e-9, 3.90625e-9), (0, 0, 0)),
               ((4, 48, 64), (3e-9, 4e-9, 4e-9), (0, 0, 0)),
               ((6, 32, 40), (3e-9, 4e-9, 4e-9), (0, 0, 0)),
               ((2, 16, 512), (3e-9, 4e-9, 4e-9), (0, 0, 0)),
-              ((1, 1024, 8), (3e-9, 4e-9, 4e-9), (0, 0, 0))]
+              ((1, 1024, 8), (3e-9, 4e-9, 4e-9), (0, 0, 0)),
+              # the longest transforms: y 2048 in 3-D, x half-lengths 1024 and 2048
+              ((2, 1024, 24), (2e-9, 2e-9, 2e-9), (0, 0, 0)),
+              ((2, 12, 1024), (2e-9, 2e-9, 2e-9), (0, 0, 0)),
+              ((1, 8, 2048), (3e-9, 4e-9, 4e-9), (0, 0, 0))]
 
 
 @pytest.mark.parametrize("n,cell,pbc", CONV_CASES)
@@ -270,6 +274,7 @@ def _problem_pair(n, cell, anis=False, zl=False, bext=(0.0, 0.0, 0.0), seed=21, 
 
 HEFF_CASES = [((1, 32, 128), (3e-9, 3.90625e-9, 3.90625e-9), False, False),
               ((8, 32, 32), (4e-9, 4e-9, 4e-9), True, False),
+              ((2, 16, 1024), (2e-9, 2e-9, 2e-9), True, False),
               ((2, 16, 48), (3e-9, 4e-9, 4e-9), False, True)]
 
 

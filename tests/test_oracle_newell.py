@@ -180,3 +180,14 @@ def test_kernel_arne_converges_to_newell():
     # Kernel_Arne's charge integration resolves slowly (~5e-4 at accuracy 6..24)
     assert errs[0] > errs[1] > errs[2] > errs[3], errs
     assert errs[-1] < 5e-5 and max(worst[1:]) < 1e-3, (errs, worst)
+
+
+def test_newell_khat_matches_kernel_fft():
+    """The memory-lean transform (octant table -> one component at a time)
+    equals the transform of the full real-space kernel."""
+    for n, d in (((3, 4, 6), SURR), ((1, 8, 12), FILM)):
+        p = O.pad_size(n)
+        a, bad_a = O.kernel_fft(p, K9d=O.kernel_newell(p, d))
+        b, bad_b = O.newell_khat(p, d)
+        np.testing.assert_allclose(b, a, rtol=0, atol=1e-14 * np.abs(a).max())
+        assert bad_b < 1e-12
