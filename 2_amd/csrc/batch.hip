@@ -42,6 +42,7 @@ struct BatchSim {
     float *mbuf[2];
     float *mpred, *tau0;
     const float *msat_map;
+    int zsmall;       // z pass as one thread per column (z_column_small)
     int tile[BP_N];   // rows (XF, XI) or columns (Y, Z, YI) per workgroup
     int ntx[BP_N];    // column tiles per line set (Y, Z, YI)
     int first[BP_N];  // first block of this simulation in each pass
@@ -109,6 +110,13 @@ bz_kernel(const BatchSim *__restrict__ sims, const BatchState *__restrict__ st,
     const BatchState &Z = st[s];
     if (!Z.active || (STAGE == 0 && !Z.need_stage0)) return;
     const int b = blockIdx.x - S.first[BP_Z], W = S.tile[BP_Z], ntx = S.ntx[BP_Z];
+    if (S.zsmall) {
+        const long col = (long)b * kBlock + threadIdx.x;
+        if (col >= (long)S.g.p1 * S.g.h2) return;
+        const int ky = (int)(col / S.g.h2), kx = (int)(col - (long)ky * S.g.h2);
+        z_column_small_any(S.g, S.kern, S.ybuf, ky, kx);
+        return;
+    }
     z_tile(S.g, S.fz, W, (b % ntx) * W, b / ntx, S.kern, S.ybuf, sm);
 }
 
@@ -326,6 +334,11 @@ int build_tables(BatchImpl &B)
             S.tile[p] = 1;
             S.ntx[p] = 0;
             if (!pass_applies(p, S.g)) continue;
+            if (p == BP_Z && S.zsmall) {
+                const int nb = (int)div_up((long)S.g.p1 * S.g.h2, kBlock);
+                maps[p].insert(maps[p].end(), (size_t)nb, s);
+                continue;
+            }
             const int v = pick_tile(p, S.g, B.cap[p]);
             S.tile[p] = v;
             const int nb = pass_blocks(p, S.g, v, &S.ntx[p]);
@@ -372,6 +385,7 @@ int load_sims(BatchImpl &B, double duration, long long nsteps)
         S.mpred = x->mpred;
         S.tau0 = x->tau0;
         S.msat_map = x->msat_map;
+        S.zsmall = z_small_applies(P.g) ? 1 : 0;
         BatchState &Z = B.hst[s];
         std::memset(&Z, 0, sizeof(Z));
         Z.t = Z.t0 = x->t;

@@ -51,6 +51,16 @@ z_kernel(DemagGeom g, LineFFT<float> fz, int W, const float *__restrict__ K,
     z_tile(g, fz, W, blockIdx.x * W, blockIdx.y, K, yb, sm);
 }
 
+// short z (z_small_applies): one thread per spectrum column
+__global__ void __launch_bounds__(kBlock)
+z_small_kernel(DemagGeom g, const float *__restrict__ K, float2 *__restrict__ yb)
+{
+    const long col = (long)blockIdx.x * kBlock + threadIdx.x;
+    if (col >= (long)g.p1 * g.h2) return;
+    const int ky = (int)(col / g.h2), kx = (int)(col - (long)ky * g.h2);
+    z_column_small_any(g, K, yb, ky, kx);
+}
+
 // ---------------------------------------------- film pass: y, multiply, y^-1
 
 __global__ void __launch_bounds__(kBlock)
@@ -141,6 +151,11 @@ static int y_and_z_passes(DemagPlanImpl *P, hipStream_t st)
     }
     if (P->fast_z) {
         MMX_TRY(fast_z(P, st));
+        mark(P, st);
+    } else if (z_small_applies(g)) {
+        z_small_kernel<<<dim3((unsigned)div_up((long)g.p1 * g.h2, kBlock)), dim3(kBlock), 0, st>>>(
+            g, P->kern, P->ybuf);
+        MMX_CHECK_LAUNCH();
         mark(P, st);
     } else {
         const int ntx_z = div_up(g.h2, P->tiles.w_z);

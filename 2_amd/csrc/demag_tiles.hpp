@@ -241,4 +241,74 @@ xi_tile(const DemagGeom &g, const LineFFT<float> &fx, int RB, long row0,
     if (MODE == EPI_CORR) block_max_to(errmax, io.err);
 }
 
+// Z transform + kernel multiply + inverse of one spectrum column (ky, kx) for
+// short z (P0 <= 8, N0 = P0/2 nonzero planes in and kept out): direct DFTs in
+// registers, one thread per column, no LDS.  Same transform pair as z_tile
+// (forward exp(-2 pi i k n / P0), unnormalised inverse).
+template <int P0, int N0>
+__device__ __forceinline__ void z_column_small(const DemagGeom &g, const float *__restrict__ K,
+                                               float2 *__restrict__ yb, int ky, int kx)
+{
+    const long cstride = (long)g.n0 * g.p1 * g.sx;
+    const long istride = (long)g.p1 * g.sx;
+    float2 *base = yb + (long)ky * g.sx + kx;
+    float2 a[3][N0];
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+#pragma unroll
+        for (int i = 0; i < N0; i++) a[c][i] = base[c * cstride + i * istride];
+    float cs[P0], sn[P0];
+#pragma unroll
+    for (int q = 0; q < P0; q++) sincospif(2.0f * (float)q / (float)P0, &sn[q], &cs[q]);
+    float2 X[3][P0];
+#pragma unroll
+    for (int kz = 0; kz < P0; kz++)
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            float2 acc = float2{0.0f, 0.0f};
+#pragma unroll
+            for (int i = 0; i < N0; i++) {
+                const int q = (kz * i) % P0;
+                acc.x += a[c][i].x * cs[q] + a[c][i].y * sn[q];
+                acc.y += a[c][i].y * cs[q] - a[c][i].x * sn[q];
+            }
+            X[c][kz] = acc;
+        }
+#pragma unroll
+    for (int kz = 0; kz < P0; kz++) {
+        const Kern6 k = kern_at(g, K, kz, ky, kx);
+        kern_apply(k, X[0][kz], X[1][kz], X[2][kz]);
+    }
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+#pragma unroll
+        for (int i = 0; i < N0; i++) {
+            float2 acc = float2{0.0f, 0.0f};
+#pragma unroll
+            for (int kz = 0; kz < P0; kz++) {
+                const int q = (kz * i) % P0;
+                acc.x += X[c][kz].x * cs[q] - X[c][kz].y * sn[q];
+                acc.y += X[c][kz].y * cs[q] + X[c][kz].x * sn[q];
+            }
+            base[c * cstride + i * istride] = acc;
+        }
+}
+
+// True when z_column_small covers the geometry.
+__host__ __device__ inline bool z_small_applies(const DemagGeom &g)
+{
+    return (g.p0 == 4 || g.p0 == 6 || g.p0 == 8) && 2 * g.n0 == g.p0;
+}
+
+// Dispatch on the z length (the caller checked z_small_applies).
+__device__ __forceinline__ void z_column_small_any(const DemagGeom &g, const float *__restrict__ K,
+                                                   float2 *__restrict__ yb, int ky, int kx)
+{
+    switch (g.p0) {
+    case 4: z_column_small<4, 2>(g, K, yb, ky, kx); break;
+    case 6: z_column_small<6, 3>(g, K, yb, ky, kx); break;
+    default: z_column_small<8, 4>(g, K, yb, ky, kx); break;
+    }
+}
+
 } // namespace mmx

@@ -245,6 +245,96 @@ def run_precision(args):
     }
 
 
+def run_batch(args):
+    """The batching configuration: 64 independent films of seeded uneven sizes
+    (32x32x1 .. 256x128x4, cell 4x4x3 nm, seeded Ms, alpha, random m, Zhang-Li
+    at 1e12 A/m^2 along x) advanced by the same simulated time, once as one
+    batch (one launch per pass for all, device step control, hipGraph
+    replays, autotuned tiles) and once one simulation after the other with
+    their own engines, on the same GPU."""
+    import numpy as np
+    import torch
+    import mmx
+    from mmx import synth
+
+    def say(*a):
+        print("[batch]", *a, file=sys.stderr, flush=True)
+
+    cfgs = synth.batch_configs(64)
+
+    def make():
+        out = []
+        for c in cfgs:
+            x = mmx.Sim(c["n"], c["cell"])
+            synth.setup_sim(x, c)
+            out.append(x)
+        return out
+
+    warm, T = 2e-13, 1e-12
+    sa = make()
+    b = mmx.Batch(sa, tries_per_graph=8)
+    t0 = time.perf_counter()
+    lds = b.tune(reps=3)
+    t_tune = time.perf_counter() - t0
+    b.run(warm)
+    steps0 = sum(x.getv("step") for x in sa)
+    evals0 = sum(x.getv("evals") for x in sa)
+    say("tuned %s in %.2f s; timing the batch" % (lds, t_tune))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    b.run(T)
+    torch.cuda.synchronize()
+    wall_b = time.perf_counter() - t0
+    steps_b = sum(x.getv("step") for x in sa) - steps0
+    evals_b = sum(x.getv("evals") for x in sa) - evals0
+
+    if args.no_sequential:
+        return {"batch_only": True, "wall_s": wall_b, "evals_per_s": evals_b / wall_b,
+                "rk_steps": int(steps_b), "tuned_lds_kb": lds}
+    sb = make()
+    for x in sb:
+        x.run(warm)
+    steps0 = sum(x.getv("step") for x in sb)
+    evals0 = sum(x.getv("evals") for x in sb)
+    say("batch %.3f s; timing the simulations one by one" % wall_b)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for x in sb:
+        x.run(T)
+    torch.cuda.synchronize()
+    wall_s = time.perf_counter() - t0
+    steps_s = sum(x.getv("step") for x in sb) - steps0
+    evals_s = sum(x.getv("evals") for x in sb) - evals0
+    dm = max(float(np.abs(np.array(x.getv("<m>")) - np.array(y.getv("<m>"))).max())
+             for x, y in zip(sa, sb))
+    cells = sum(int(np.prod(c["n"])) for c in cfgs)
+    info = b.info()
+    return {
+        "metric": "H_eff evals/s + simulated ns/s at 256^2x64 FP32; surrogate vs exact, same GPU",
+        "value": evals_b / wall_b, "unit": "H_eff evals/s summed over 64 simulations",
+        "n_gpus": 1, "steps": int(steps_b), "warmup": 0,
+        "ms_per_step": 1e3 * wall_b / max(1, steps_b / 64), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "batch of 64 films 32x32x1..256x128x4 (seeded sizes, Ms, alpha, "
+                               "random m), cell 4x4x3 nm, Zhang-Li 1e12 A/m^2 along x, P 0.56, "
+                               "xi 0.05; %d cells in total; each advanced %.0e s" % (cells, T),
+                   "steps_are": "accepted RK12 steps, summed over the simulations"},
+        "roofline": {"bound": "latency", "achieved": None, "peak": None, "unit": None,
+                     "frac": None, "traffic": None,
+                     "note": "working set per simulation < 50 MB (Infinity Cache); judged by "
+                             "launches per try and the speed-up over one-by-one stepping"},
+        "cpu_baseline": None,
+        "batch": {"wall_s": wall_b, "rk_steps": int(steps_b), "evals": int(evals_b),
+                  "simulated_ns_per_s_summed": 64 * T * 1e9 / wall_b,
+                  "launches_per_try": 11, "tries_per_graph": 8, "tuned_lds_kb": lds,
+                  "tune_s": t_tune, "blocks_per_pass": {k: v["blocks"] for k, v in info.items()}},
+        "one_by_one": {"wall_s": wall_s, "rk_steps": int(steps_s), "evals": int(evals_s),
+                       "evals_per_s": evals_s / wall_s},
+        "speedup_evals_per_s": (evals_b / wall_b) / (evals_s / wall_s),
+        "max_abs_d<m>_batch_vs_one_by_one": dm,
+    }
+
+
 def main():
     if len(sys.argv) > 1 and sys.argv[1] == "--summarize":
         print(summarize_profiles(*sys.argv[2:5]))
@@ -255,10 +345,12 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--rk-steps", type=int, default=1000)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=("bulk", "precision"), default="bulk",
+    ap.add_argument("--workload", choices=("bulk", "precision", "batch"), default="bulk",
                     help="bulk: 256x256x64 (the headline line); precision: 1024x1024x32 film "
                          "with the FP64 Newell kernel and per-field errors against the FP64 "
                          "oracle")
+    ap.add_argument("--no-sequential", action="store_true",
+                    help="batch workload: skip the one-by-one comparison run")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run one 1 GiB device-to-device copy (known bytes) so that "
                          "profiler byte counters can be calibrated in the same run")
@@ -275,6 +367,9 @@ def main():
     mmx.check(mmx.lib().mmx_init(0))
     if args.workload == "precision":
         print(json.dumps(run_precision(args)))
+        return
+    if args.workload == "batch":
+        print(json.dumps(run_batch(args)))
         return
     cfg = synth.config(2)
     sim = mmx.Sim(cfg["n"], cfg["cell"])
