@@ -35,11 +35,30 @@ struct mmx_sim {
     unsigned int *derr = nullptr;
     unsigned int *herr = nullptr; // pinned
     hipStream_t st = nullptr;
+
+    // learned inner-stage field (borrowed; mmx_sim_set_surrogate)
+    mmx_surrogate *sur = nullptr;
+    bool sur_on = false;
+    double sur_tol = 1e-3;
+    int sur_check = 8;
+    long sur_checks = 0, sur_fallbacks = 0, sur_evals = 0;
+    double sur_residual = 0.0;
 };
 
 namespace mmx {
 // Msat/gamma set, demag plan built, m normalised (queued on s->st).
 int sim_prepare(mmx_sim *s);
+// Surrogate field evaluation (surrogate.hip): at m0, or at the RK predictor
+// normalize(m0 + h tau0) written to mpred when tau0 is given; the H_demag
+// estimate goes into the epilogue of `mode` (and to hd_out if not NULL).
+int surrogate_eval(mmx_surrogate *S, int mode, const float *m0, const float *tau0, float h,
+                   float *mpred, const FieldParams &fp, const EpilogueIO &io, float *hd_out,
+                   hipStream_t st);
+// ||a - b|| / ||c|| over n floats (synchronises).
+int surrogate_rel_l2(mmx_surrogate *S, const float *a, const float *b, const float *c, long n,
+                     hipStream_t st, double *rel);
+// the surrogate's H_demag buffer (3N floats)
+float *surrogate_hd_buffer(mmx_surrogate *S);
 // The epilogue parameters of the simulation's current settings (H_eff mode).
 FieldParams sim_field_params(const mmx_sim *s);
 } // namespace mmx

@@ -13,8 +13,8 @@ Two levels, as in the reference:
 """
 from ._lib import MMXError, lib, load_library, check, LIB_PATH, EXPORTED_SYMBOLS  # noqa: F401
 from ._lib import MMX_KERNEL_ARNE, MMX_KERNEL_NEWELL  # noqa: F401
-from .sim import Sim, DemagPlan, Reductor, Batch  # noqa: F401
+from .sim import Sim, DemagPlan, Reductor, Batch, Surrogate  # noqa: F401
 from . import kernels  # noqa: F401
 
-__all__ = ["MMXError", "lib", "load_library", "check", "Sim", "DemagPlan", "Reductor", "Batch",
+__all__ = ["MMXError", "lib", "load_library", "check", "Sim", "DemagPlan", "Reductor", "Batch", "Surrogate",
            "kernels", "LIB_PATH", "EXPORTED_SYMBOLS", "MMX_KERNEL_ARNE", "MMX_KERNEL_NEWELL"]

@@ -62,6 +62,16 @@ _SIGS = {
     "mmx_sim_run": (_i, [_vp, _d]),
     "mmx_sim_try_step": (_i, [_vp, _d, C.POINTER(_d)]),
     "mmx_sim_time_passes": (_i, [_vp, _i, C.POINTER(_d), C.POINTER(_i)]),
+    "mmx_surrogate_create": (_i, [C.POINTER(_vp), _vp, C.c_ulonglong]),
+    "mmx_surrogate_destroy": (_i, [_vp]),
+    "mmx_surrogate_num_params": (_i, [_vp, C.POINTER(_i)]),
+    "mmx_surrogate_get_params": (_i, [_vp, _vp, _i]),
+    "mmx_surrogate_set_params": (_i, [_vp, _vp, _i]),
+    "mmx_surrogate_get_grads": (_i, [_vp, _vp, _i]),
+    "mmx_surrogate_train_step": (_i, [_vp, _fp, _f, C.POINTER(_d)]),
+    "mmx_surrogate_net": (_i, [_vp, _vp, _vp, _vp, C.POINTER(_d)]),
+    "mmx_surrogate_predict_b": (_i, [_vp, _fp, _fp, _i]),
+    "mmx_sim_set_surrogate": (_i, [_vp, _vp, _d, _i]),
     "mmx_batch_create": (_i, [C.POINTER(_vp), C.POINTER(_vp), _i]),
     "mmx_batch_destroy": (_i, [_vp]),
     "mmx_batch_run": (_i, [_vp, _d]),

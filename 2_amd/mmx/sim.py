@@ -44,6 +44,10 @@ class Sim:
     def load(self, module):
         check(lib().mmx_sim_load(self._h, module.encode()))
 
+    def set_surrogate(self, sur, tol=1e-3, check_every=8):
+        check(lib().mmx_sim_set_surrogate(self._h, None if sur is None else sur._h, float(tol),
+                                          int(check_every)))
+
     def set_kernel(self, kernel_type=MMX_KERNEL_ARNE, accuracy=6):
         check(lib().mmx_sim_set_kernel(self._h, int(kernel_type), int(accuracy)))
 
@@ -112,6 +116,85 @@ class Sim:
         err = C.c_double(0)
         check(lib().mmx_sim_try_step(self._h, float(dt), C.byref(err)))
         return err.value
+
+
+class Surrogate:
+    """Learned demag correction for one simulation's grid (mmx_surrogate).
+
+    Parameters are a flat float32 vector: per layer l = 0..3 the weights
+    (3, 3, Cin, Cout) then the biases (Cout,), channels 32, 32, 32, 32, 16."""
+
+    CH = (32, 32, 32, 32, 16)
+
+    def __init__(self, sim, seed=0):
+        self.sim = sim
+        h = C.c_void_p()
+        check(lib().mmx_surrogate_create(C.byref(h), sim._h, int(seed)))
+        self._h = h
+        n = C.c_int()
+        check(lib().mmx_surrogate_num_params(h, C.byref(n)))
+        self.nparams = n.value
+        self.n = sim.n
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().mmx_surrogate_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def get_params(self):
+        out = np.empty(self.nparams, np.float32)
+        check(lib().mmx_surrogate_get_params(self._h, out.ctypes.data, self.nparams))
+        return out
+
+    def set_params(self, flat):
+        flat = np.ascontiguousarray(flat, np.float32)
+        assert flat.size == self.nparams
+        check(lib().mmx_surrogate_set_params(self._h, flat.ctypes.data, self.nparams))
+
+    def get_grads(self):
+        out = np.empty(self.nparams, np.float32)
+        check(lib().mmx_surrogate_get_grads(self._h, out.ctypes.data, self.nparams))
+        return out
+
+    @classmethod
+    def unflatten(cls, flat):
+        Ws, bs, o = [], [], 0
+        for l in range(4):
+            ci, co = cls.CH[l], cls.CH[l + 1]
+            Ws.append(np.asarray(flat[o:o + 9 * ci * co], np.float64).reshape(3, 3, ci, co))
+            o += 9 * ci * co
+            bs.append(np.asarray(flat[o:o + co], np.float64))
+            o += co
+        return Ws, bs
+
+    @classmethod
+    def flatten(cls, Ws, bs):
+        return np.concatenate([np.concatenate([w.ravel(), b.ravel()]) for w, b in zip(Ws, bs)]
+                              ).astype(np.float32)
+
+    def net(self, x, tgt=None):
+        """x: (n1, n2, 32) -> out (n1, n2, 16); with tgt (n1, n2, 3 n0) also the loss."""
+        x = np.ascontiguousarray(x, np.float32)
+        out = np.empty(x.shape[:2] + (16,), np.float32)
+        loss = C.c_double(0)
+        t = None if tgt is None else np.ascontiguousarray(tgt, np.float32)
+        check(lib().mmx_surrogate_net(self._h, x.ctypes.data, None if t is None else t.ctypes.data,
+                                      out.ctypes.data, C.byref(loss)))
+        return out if tgt is None else (out, loss.value)
+
+    def train_step(self, m_ptr, lr):
+        loss = C.c_double(0)
+        check(lib().mmx_surrogate_train_step(self._h, m_ptr, float(lr), C.byref(loss)))
+        return loss.value
+
+    def predict_b(self, m_ptr, b_ptr, coarse_only=False):
+        check(lib().mmx_surrogate_predict_b(self._h, m_ptr, b_ptr, 1 if coarse_only else 0))
 
 
 class Batch:

@@ -75,6 +75,70 @@ def smooth_texture_m(seed, n_internal, cell_internal, nmodes=16):
     return m.astype(np.float32)
 
 
+def vortex_m(seed, n_internal, cell_internal):
+    """A vortex per layer stack: seeded centre, chirality, polarity and core
+    radius; in-plane curl around the centre, out-of-plane core."""
+    n0, n1, n2 = n_internal
+    u = uniform(seed, 6, offset=7 << 32)
+    y = (np.arange(n1) + 0.5) * cell_internal[1]
+    x = (np.arange(n2) + 0.5) * cell_internal[2]
+    yc = (0.2 + 0.6 * u[0]) * n1 * cell_internal[1]
+    xc = (0.2 + 0.6 * u[1]) * n2 * cell_internal[2]
+    chi = 1.0 if u[2] < 0.5 else -1.0
+    pol = 1.0 if u[3] < 0.5 else -1.0
+    rc = (5.0 + 15.0 * u[4]) * 1e-9
+    dy = y[:, None] - yc
+    dx = x[None, :] - xc
+    r = np.sqrt(dx * dx + dy * dy) + 1e-30
+    m = np.zeros((3, n0, n1, n2))
+    m[2] = (-chi * dy / r)[None]                 # user x
+    m[1] = (chi * dx / r)[None]                  # user y
+    m[0] = (pol * np.exp(-(r * r) / (rc * rc)))[None]
+    m /= np.sqrt((m * m).sum(axis=0))
+    return m.astype(np.float32)
+
+
+def domain_wall_m(seed, n_internal, cell_internal):
+    """A straight Neel or Bloch wall: seeded normal direction, position, width
+    and domain axis (in-plane)."""
+    n0, n1, n2 = n_internal
+    u = uniform(seed, 6, offset=9 << 32)
+    y = (np.arange(n1) + 0.5) * cell_internal[1]
+    x = (np.arange(n2) + 0.5) * cell_internal[2]
+    th = 2 * np.pi * u[0]
+    ny_, nx_ = np.sin(th), np.cos(th)
+    pos = (0.3 + 0.4 * u[1]) * (abs(nx_) * n2 * cell_internal[2] + abs(ny_) * n1 * cell_internal[1])
+    width = (5.0 + 20.0 * u[2]) * 1e-9
+    s = nx_ * x[None, :] + ny_ * y[:, None]
+    s = s - s.min() - pos
+    ph = 2 * np.pi * u[3]
+    ax, ay = np.cos(ph), np.sin(ph)              # domain axis (user x, y)
+    m = np.zeros((3, n0, n1, n2))
+    t, sh = np.tanh(s / width), 1.0 / np.cosh(s / width)
+    m[2] = (ax * t)[None]
+    m[1] = (ay * t)[None]
+    if u[4] < 0.5:                               # Neel: rotates in-plane
+        m[2] += (-ay * sh)[None]
+        m[1] += (ax * sh)[None]
+    else:                                        # Bloch: rotates out of plane
+        m[0] = sh[None] * np.ones((n0, 1, 1))
+    m /= np.sqrt((m * m).sum(axis=0))
+    return m.astype(np.float32)
+
+
+def training_texture(k, n_internal, cell_internal):
+    """Texture number k of the surrogate's training set: vortices, domain
+    walls, smooth and rough random states in turn."""
+    kind = k % 4
+    if kind == 0:
+        return vortex_m(1000 + k, n_internal, cell_internal)
+    if kind == 1:
+        return domain_wall_m(2000 + k, n_internal, cell_internal)
+    if kind == 2:
+        return smooth_texture_m(3000 + k, n_internal, cell_internal, nmodes=8)
+    return random_unit_m(4000 + k, n_internal)
+
+
 def to_internal(v):
     """User (x, y, z) triple -> internal (z, y, x)."""
     return (v[2], v[1], v[0])

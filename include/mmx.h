@@ -255,6 +255,47 @@ int mmx_sim_time_passes(mmx_sim *sim, int reps, double *ms, int *npass);
 int mmx_sim_try_step(mmx_sim *sim, double dt, double *err);
 
 /* ------------------------------------------------------------------------
+ * Learned demag correction (no reference counterpart).  For a film of n0 in
+ * {1, 2, 4} layers with n1, n2 multiples of 16: the exact demag field of the
+ * 2x-coarsened grid, upsampled, plus a 4-layer 3x3 convolutional net (width
+ * 32, BF16 matrix cores, FP32 accumulation) that predicts the fine-grid
+ * residual from m and the coarse field.  Trained on the GPU against the exact
+ * field.  Used by a simulation (mmx_sim_set_surrogate) for the RK12
+ * corrector's field; the first stage stays exact and judges the surrogate
+ * every check_every steps.
+ * ---------------------------------------------------------------------- */
+typedef struct mmx_surrogate mmx_surrogate;
+
+/* For the grid, cell and Msat of `sim` (which must outlive it); weights
+ * He-initialised from `seed`. */
+int mmx_surrogate_create(mmx_surrogate **sur, mmx_sim *sim, unsigned long long seed);
+int mmx_surrogate_destroy(mmx_surrogate *sur);
+/* Parameters: per layer l = 0..3 the weights [3][3][Cin][Cout] then the
+ * biases [Cout], channels 32, 32, 32, 32, 16; FP32. */
+int mmx_surrogate_num_params(const mmx_surrogate *sur, int *count);
+int mmx_surrogate_get_params(mmx_surrogate *sur, float *host, int count);
+int mmx_surrogate_set_params(mmx_surrogate *sur, const float *host, int count);
+/* Gradients of the last mmx_surrogate_train_step / mmx_surrogate_net. */
+int mmx_surrogate_get_grads(mmx_surrogate *sur, float *host, int count);
+/* One training step on the magnetisation m_dev (device, 3*N floats): exact
+ * fine field as the target, forward, MSE loss, backward, Adam with learning
+ * rate lr (lr = 0: gradients only).  *loss (may be NULL) = the loss before
+ * the update. */
+int mmx_surrogate_train_step(mmx_surrogate *sur, const float *m_dev, float lr, double *loss);
+/* The net alone on a host input x[n1*n2][32] (rounded to bf16): out[n1*n2][16]
+ * (FP32); with a target tgt[n1*n2][3*n0] also the loss and the gradients. */
+int mmx_surrogate_net(mmx_surrogate *sur, const float *x_host, const float *tgt_host,
+                      float *out_host, double *loss);
+/* The surrogate's B (tesla, device 3*N floats) at m_dev; coarse_only = 1
+ * leaves the learned correction out (the method without the net). */
+int mmx_surrogate_predict_b(mmx_surrogate *sur, const float *m_dev, float *b_dev, int coarse_only);
+/* Use (sur != NULL) or stop using the surrogate in mmx_sim_step; tol: relative
+ * L2 residual of the demag field above which the solver falls back to the
+ * exact field (getv: surrogate_on, surrogate_residual, surrogate_checks,
+ * surrogate_fallbacks, surrogate_evals). */
+int mmx_sim_set_surrogate(mmx_sim *sim, mmx_surrogate *sur, double tol, int check_every);
+
+/* ------------------------------------------------------------------------
  * Batches: many independent simulations of uneven sizes stepped together
  * (no reference counterpart; the reference runs one simulation per process,
  * Engine.Run engine.go:489-524).  One launch per demag pass covers every
