@@ -3,17 +3,21 @@
 // No reference counterpart (the reference has no learned component).  For a
 // film of n0 <= 5 layers on an n1 x n2 in-plane grid (internal z,y,x order):
 //
-//   coarse m  = means of 2x2x2 blocks of m (2x2 when n0 == 1);
+//   coarse m  = means of 2x2 in-plane blocks of m, layers kept (coarsening the
+//               few layers of a film loses most of the field: 92 % error on a
+//               random state against 23 % for in-plane coarsening);
 //   coarse B  = exact demag of the coarse grid (its own demag plan);
-//   net input = per in-plane pixel, channel c*n0+i = m_c of layer i and
-//               3*n0 + c*n0+i = the upsampled coarse B_c / (mu0 Ms), in bf16;
+//   net input = per in-plane pixel, channel c*n0+i = m_c of layer i,
+//               3*n0 + c*n0+i = the upsampled coarse B_c / (mu0 Ms), and
+//               6*n0, 6*n0+1 = the position in the coarse cell (+-1 for odd /
+//               even y, x: the residual operator has period 2), in bf16;
 //   net       = 4 conv 3x3 layers 32 -> 32 -> 32 -> 32 -> 16 (ReLU after the
 //               first three), BF16 MFMA (v_mfma_f32_16x16x16_bf16) with FP32
 //               accumulation, as implicit GEMMs over 16 x 8 pixel tiles;
 //   B_fine   ~= upsampled coarse B + mu0 Ms * out[c*n0+i].
 //
-// Training (on the GPU, by these kernels): MSE of the residual against the
-// exact fine field, backward-data as the same convolution kernel with flipped
+// Training (on the GPU, by these kernels): relative squared error of the
+// residual against the exact fine field (sum d^2 / sum target^2 per sample), backward-data as the same convolution kernel with flipped
 // and transposed weights and a ReLU-mask epilogue, weight gradients as MFMA
 // products over the pixel tiles with per-tile partial sums reduced in a fixed
 // order (deterministic), Adam on FP32 master weights.  The CPU oracle
@@ -124,6 +128,8 @@ __device__ __forceinline__ float input_channel(const SurGeom &s, const float *__
         const int q = ch - nm, c = q / s.n0, i = q - c * s.n0;
         return bc[c * s.Nc + coarse_of(s, i, y, x)] * s.inv_mu0ms;
     }
+    if (ch == 2 * nm) return (y & 1) ? 1.0f : -1.0f;
+    if (ch == 2 * nm + 1) return (x & 1) ? 1.0f : -1.0f;
     return 0.0f;
 }
 
@@ -312,33 +318,60 @@ sur_reduce_parts(const float *__restrict__ part, int ntile, int len, float *__re
     grad[e] = s;
 }
 
-// d loss / d out (bf16) and per-block partial sums of the squared error, with
-// the target formed from the exact fine B and the coarse B
+__device__ __forceinline__ float target_at(const SurGeom &s, const float *__restrict__ bfine,
+                                           const float *__restrict__ bc,
+                                           const float *__restrict__ tgt_direct, long p, int ch)
+{
+    const int nused = 3 * s.n0;
+    if (tgt_direct) return tgt_direct[p * nused + ch];
+    const int c = ch / s.n0, i = ch - c * s.n0;
+    const int y = (int)(p / s.n2), x = (int)(p - (long)y * s.n2);
+    const long I = ((long)i * s.n1 + y) * s.n2 + x;
+    return (bfine[c * s.N + I] - bc[c * s.Nc + coarse_of(s, i, y, x)]) * s.inv_mu0ms;
+}
+
+// per-block partial sums of target^2
+__global__ void __launch_bounds__(kBlock)
+sur_target_sq(SurGeom s, const float *__restrict__ bfine, const float *__restrict__ bc,
+              const float *__restrict__ tgt_direct, float *__restrict__ part)
+{
+    __shared__ float red[kBlock / 64];
+    const long P = (long)s.n1 * s.n2;
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x; // pixel * 16 + channel
+    float v2 = 0.0f;
+    if (t < P * 16 && (int)(t & 15) < 3 * s.n0) {
+        const float v = target_at(s, bfine, bc, tgt_direct, t >> 4, (int)(t & 15));
+        v2 = v * v;
+    }
+    v2 = wave_sum(v2);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v2;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float s2 = 0.0f;
+        for (int w = 0; w < kBlock / 64; w++) s2 += red[w];
+        part[blockIdx.x] = s2;
+    }
+}
+
+// d loss / d out = 2 (out - target) / sum target^2 (bf16) and per-block
+// partial sums of the squared error; *t2 = sum target^2 (device)
 __global__ void __launch_bounds__(kBlock)
 sur_loss(SurGeom s, const float *__restrict__ out, const float *__restrict__ bfine,
-         const float *__restrict__ bc, const float *__restrict__ tgt_direct, float gscale,
-         bf16_t *__restrict__ gout, float *__restrict__ part)
+         const float *__restrict__ bc, const float *__restrict__ tgt_direct,
+         const float *__restrict__ t2, bf16_t *__restrict__ gout, float *__restrict__ part)
 {
     __shared__ float red[kBlock / 64];
     const long P = (long)s.n1 * s.n2;
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x; // pixel * 16 + channel
     const int nused = 3 * s.n0;
     float e2 = 0.0f;
+    const float gscale = 2.0f / fmaxf(*t2, 1e-30f);
     if (t < P * 16) {
         const int ch = (int)(t & 15);
         const long p = t >> 4;
         float g = 0.0f;
         if (ch < nused) {
-            float tg;
-            if (tgt_direct) {
-                tg = tgt_direct[p * nused + ch];
-            } else {
-                const int c = ch / s.n0, i = ch - c * s.n0;
-                const int y = (int)(p / s.n2), x = (int)(p - (long)y * s.n2);
-                const long I = ((long)i * s.n1 + y) * s.n2 + x;
-                tg = (bfine[c * s.N + I] - bc[c * s.Nc + coarse_of(s, i, y, x)]) * s.inv_mu0ms;
-            }
-            const float d = out[t] - tg;
+            const float d = out[t] - target_at(s, bfine, bc, tgt_direct, p, ch);
             e2 = d * d;
             g = gscale * d;
         }
@@ -522,9 +555,10 @@ int net_backward(mmx_surrogate *S, const float *tgt_direct, double *loss, hipStr
 {
     const long n16 = S->P * 16;
     const int nlb = (int)div_up(n16, kBlock);
-    const int nused = 3 * S->s.n0;
-    const float gscale = 2.0f / (float)(S->P * nused);
-    sur_loss<<<dim3(nlb), dim3(kBlock), 0, st>>>(S->s, S->out, S->bfine, S->bc, tgt_direct, gscale,
+    float *t2 = S->loss_part + nlb; // sum target^2 (one float after the partials)
+    sur_target_sq<<<dim3(nlb), dim3(kBlock), 0, st>>>(S->s, S->bfine, S->bc, tgt_direct, S->loss_part);
+    sur_reduce_parts<<<dim3(1), dim3(kBlock), 0, st>>>(S->loss_part, nlb, 1, t2);
+    sur_loss<<<dim3(nlb), dim3(kBlock), 0, st>>>(S->s, S->out, S->bfine, S->bc, tgt_direct, t2,
                                                  S->gout, S->loss_part);
     const dim3 grid(S->W / 16, S->H / 8), blk(kBlock);
     const dim3 wgrid(S->W / 16, S->H / 16);
@@ -548,12 +582,13 @@ int net_backward(mmx_surrogate *S, const float *tgt_direct, double *loss, hipStr
     }
     MMX_CHECK_LAUNCH();
     if (loss) {
-        std::vector<float> lp(nlb);
-        MMX_HIP(hipMemcpyAsync(lp.data(), S->loss_part, sizeof(float) * nlb, hipMemcpyDeviceToHost, st));
+        std::vector<float> lp(nlb + 1);
+        MMX_HIP(hipMemcpyAsync(lp.data(), S->loss_part, sizeof(float) * (nlb + 1),
+                               hipMemcpyDeviceToHost, st));
         MMX_HIP(hipStreamSynchronize(st));
         double sum = 0.0;
-        for (float v : lp) sum += v;
-        *loss = sum / (double)(S->P * nused);
+        for (int b = 0; b < nlb; b++) sum += lp[b];
+        *loss = lp[nlb] > 0 ? sum / (double)lp[nlb] : 0.0;
     }
     return MMX_OK;
 }
@@ -642,15 +677,15 @@ int mmx_surrogate_create(mmx_surrogate **sur, mmx_sim *sim, unsigned long long s
     MMX_REQUIRE(sur && sim, MMX_ERR_ARG, "mmx_surrogate_create: NULL argument");
     MMX_TRY(sim_prepare(sim));
     const int n0 = sim->n[0], n1 = sim->n[1], n2 = sim->n[2];
-    MMX_REQUIRE((n0 == 1 || n0 % 2 == 0) && n0 <= 4 && n1 % 16 == 0 && n2 % 16 == 0, MMX_ERR_SIZE,
-                "surrogate: needs n0 in {1, 2, 4} and n1, n2 multiples of 16 (got %d %d %d)", n0, n1, n2);
+    MMX_REQUIRE(n0 >= 1 && n0 <= 5 && n1 % 16 == 0 && n2 % 16 == 0, MMX_ERR_SIZE,
+                "surrogate: needs 1 <= n0 <= 5 and n1, n2 multiples of 16 (got %d %d %d)", n0, n1, n2);
     MMX_REQUIRE(!sim->pbc[0] && !sim->pbc[1] && !sim->pbc[2], MMX_ERR_ARG,
                 "surrogate: periodic grids are not supported");
     mmx_surrogate *S = new mmx_surrogate();
     S->sim = sim;
     SurGeom &s = S->s;
     s.n0 = n0; s.n1 = n1; s.n2 = n2;
-    s.f0 = n0 > 1 ? 2 : 1;
+    s.f0 = 1; // the layers are kept
     s.c0 = n0 / s.f0; s.c1 = n1 / 2; s.c2 = n2 / 2;
     s.N = (long)n0 * n1 * n2;
     s.Nc = (long)s.c0 * s.c1 * s.c2;
@@ -683,7 +718,7 @@ int mmx_surrogate_create(mmx_surrogate **sur, mmx_sim *sim, unsigned long long s
     if (rc == MMX_OK) rc = dalloc(&S->out, P * 16);
     if (rc == MMX_OK) rc = dalloc(&S->tgt, P * 3 * n0);
     if (rc == MMX_OK) rc = dalloc(&S->part, (size_t)ntile * (9 * 32 * 32 + 32));
-    if (rc == MMX_OK) rc = dalloc(&S->loss_part, (size_t)div_up(P * 16, kBlock));
+    if (rc == MMX_OK) rc = dalloc(&S->loss_part, (size_t)div_up(P * 16, kBlock) + 1);
     if (rc == MMX_OK) rc = dalloc(&S->norm_part, 2 * 512);
     if (rc == MMX_OK) rc = dalloc(&S->params, S->np);
     if (rc == MMX_OK) rc = dalloc(&S->grads, S->np);
@@ -692,13 +727,13 @@ int mmx_surrogate_create(mmx_surrogate **sur, mmx_sim *sim, unsigned long long s
     for (int l = 0; l < 4 && rc == MMX_OK; l++) rc = dalloc(&S->wt[l], 9 * kSurCh[l] * kSurCh[l + 1]);
     for (int l = 0; l < 4 && rc == MMX_OK; l++) rc = dalloc(&S->wbt[l], 9 * kSurCh[l] * kSurCh[l + 1]);
     if (rc == MMX_OK) {
-        // He initialisation (first layer over its 6 n0 used inputs), a small
-        // last layer, zero biases
+        // He initialisation (first layer over its 6 n0 + 2 used inputs), a
+        // small last layer, zero biases
         std::vector<float> w(S->np, 0.0f);
         std::mt19937_64 gen(seed);
         std::normal_distribution<double> nd(0.0, 1.0);
         for (int l = 0; l < 4; l++) {
-            const int fan = 9 * (l == 0 ? 6 * n0 : kSurCh[l]);
+            const int fan = 9 * (l == 0 ? 6 * n0 + 2 : kSurCh[l]);
             const double sd = std::sqrt(2.0 / fan) * (l == 3 ? 0.1 : 1.0);
             for (int e = 0; e < 9 * kSurCh[l] * kSurCh[l + 1]; e++) w[S->off_w[l] + e] = (float)(sd * nd(gen));
         }

@@ -9,11 +9,12 @@ import numpy as np
 from . import synth
 
 
-def train(sur, sim, nsamples=2000, steps_per_sample=2, samples_per_trajectory=100, lr=1e-3,
+def train(sur, sim, nsamples=2000, steps_per_sample=2, samples_per_trajectory=100, lr=5e-4,
           lr_final=1e-4, alpha=0.1, log=None):
     """Trains `sur` (an mmx.Surrogate of `sim`'s grid) in place.  `sim` is used
     to run the exact trajectories (its m, t and solver state are overwritten).
-    Returns the list of (sample, loss)."""
+    Learning rates above ~1e-3 let the ReLU layers die (the output collapses
+    to zero residual).  Returns the list of (sample, loss)."""
     n = sim.n
     cell = sim.cell
     hist = []

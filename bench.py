@@ -366,7 +366,7 @@ def run_surrogate(args):
     sur = mmx.Surrogate(trainer, seed=5)
     t0 = time.perf_counter()
     hist = T.train(sur, trainer, nsamples=args.train_samples, steps_per_sample=2,
-                   samples_per_trajectory=200, lr=2e-3, lr_final=1e-4, alpha=0.1, log=say)
+                   samples_per_trajectory=100, lr=5e-4, lr_final=5e-5, alpha=0.1, log=say)
     t_train = time.perf_counter() - t0
     say("trained on %d pairs in %.1f s, loss %.3e -> %.3e" % (
         len(hist), t_train, np.mean([l for _, l in hist[:50]]), np.mean([l for _, l in hist[-50:]])))
@@ -388,17 +388,27 @@ def run_surrogate(args):
         ec = float(np.linalg.norm(bco.cpu().numpy() - e) / np.linalg.norm(e))
         gains.append({"coarse_only": ec, "with_net": en})
 
-    def run(use_sur):
+    def run(use_sur, tol=1e-3, m0=None, bext=None, alpha=None, pert=0.0, T=None):
         x = make()
+        if m0 is not None:
+            x.setarray("m", m0)
+        if bext is not None:
+            x.setv("B_ext", bext)
+        if alpha is not None:
+            x.setv("alpha", alpha)
+        if pert:
+            m = x.getarray("m").astype(np.float64)
+            m += pert * np.random.default_rng(1).normal(size=m.shape)
+            x.setarray("m", (m / np.sqrt((m * m).sum(axis=0))).astype(np.float32))
         s2 = None
         if use_sur:
             s2 = mmx.Surrogate(x, seed=5)
             s2.set_params(sur.get_params())
-            x.set_surrogate(s2, tol=1e-3, check_every=8)
+            x.set_surrogate(s2, tol=tol, check_every=8)
         x.step()
         torch.cuda.synchronize()
         ts = time.perf_counter()
-        x.run(args.sim_time)
+        x.run(T if T is not None else args.sim_time)
         torch.cuda.synchronize()
         wall = time.perf_counter() - ts
         out = {"wall_s": wall, "simulated_ns_per_s": x.getv("t") * 1e9 / wall,
@@ -406,12 +416,59 @@ def run_surrogate(args):
         if use_sur:
             out.update({k: x.getv("surrogate_" + k) for k in ("on", "residual", "checks",
                                                               "fallbacks", "evals")})
-        say("%s: %s" % ("surrogate" if use_sur else "exact", out))
-        return out, x, s2
+            out["tolerance"] = tol
+        return out
 
-    ex_run, _, _ = run(False)
-    su_run, _, _ = run(True)
-    dm = float(np.abs(np.array(ex_run["<m>"]) - np.array(su_run["<m>"])).max())
+    def dmax(a, b):
+        return float(np.abs(np.array(a["<m>"]) - np.array(b["<m>"])).max())
+
+    # (a) the stated case: seeded random m, no field
+    ex_run = run(False)
+    say("exact: %s" % ex_run)
+    chaos = run(False, pert=1e-7)
+    say("exact, m perturbed by 1e-7: |d<m>| %.3e" % dmax(ex_run, chaos))
+    su_run = run(True)
+    say("surrogate (tol 1e-3): %s" % su_run)
+    dm = dmax(ex_run, su_run)
+    # (b) deterministic dynamics: a relaxed vortex gyrating under 5 mT in-plane
+    relax = make()
+    relax.setarray("m", synth.vortex_m(77, n, cell))
+    relax.setv("alpha", 0.5)
+    relax.run(3e-10)
+    mv = relax.getarray("m")
+    bx = tuple(synth.to_internal((5e-3, 0.0, 0.0)))
+    gy = {"exact": run(False, m0=mv, bext=bx)}
+    gy["surrogate_tol_1e-3"] = run(True, m0=mv, bext=bx)
+    gy["surrogate_tol_0.5"] = run(True, tol=0.5, m0=mv, bext=bx)
+    for k in ("surrogate_tol_1e-3", "surrogate_tol_0.5"):
+        gy[k]["max_abs_d<m>_vs_exact"] = dmax(gy["exact"], gy[k])
+        gy[k]["speedup_simulated_ns_per_s"] = (gy[k]["simulated_ns_per_s"] /
+                                               gy["exact"]["simulated_ns_per_s"])
+    say("vortex gyration: %s" % gy)
+    # field evaluations alone
+    evals = {}
+    for name, f in (("exact", lambda x: x.update("H_eff")),):
+        x = make()
+        f(x)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        for _ in range(200):
+            f(x)
+        torch.cuda.synchronize()
+        evals[name + "_eval_us"] = (time.perf_counter() - t1) / 200 * 1e6
+    x = make()
+    dmm = torch.from_numpy(x.getarray("m")).cuda()
+    bb = torch.empty_like(dmm)
+    for mode, key in ((True, "coarse_field_us"), (False, "coarse_plus_net_us")):
+        sur.predict_b(dmm.data_ptr(), bb.data_ptr(), coarse_only=mode)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        for _ in range(200):
+            sur.predict_b(dmm.data_ptr(), bb.data_ptr(), coarse_only=mode)
+        torch.cuda.synchronize()
+        evals[key] = (time.perf_counter() - t1) / 200 * 1e6
+    say("evaluation costs %s" % evals)
+
     return {
         "metric": "H_eff evals/s + simulated ns/s at 256^2x64 FP32; surrogate vs exact, same GPU",
         "value": su_run["simulated_ns_per_s"], "unit": "simulated ns/s (surrogate-assisted RK12)",
@@ -433,6 +490,9 @@ def run_surrogate(args):
         "exact": ex_run, "surrogate": su_run,
         "speedup_simulated_ns_per_s": su_run["simulated_ns_per_s"] / ex_run["simulated_ns_per_s"],
         "max_abs_d<m>_after_run": dm,
+        "chaos_baseline_exact_vs_exact_perturbed_1e-7": dmax(ex_run, chaos),
+        "vortex_gyration_5mT": gy,
+        "evaluation_costs": evals,
         "training": {"pairs": len(hist), "seconds": t_train,
                      "loss_first50": float(np.mean([l for _, l in hist[:50]])),
                      "loss_last50": float(np.mean([l for _, l in hist[-50:]]))},

@@ -255,9 +255,10 @@ int mmx_sim_time_passes(mmx_sim *sim, int reps, double *ms, int *npass);
 int mmx_sim_try_step(mmx_sim *sim, double dt, double *err);
 
 /* ------------------------------------------------------------------------
- * Learned demag correction (no reference counterpart).  For a film of n0 in
- * {1, 2, 4} layers with n1, n2 multiples of 16: the exact demag field of the
- * 2x-coarsened grid, upsampled, plus a 4-layer 3x3 convolutional net (width
+ * Learned demag correction (no reference counterpart).  For a film of
+ * 1 <= n0 <= 5 layers with n1, n2 multiples of 16: the exact demag field of
+ * the grid coarsened 2x in the film plane, upsampled, plus a 4-layer 3x3
+ * convolutional net (width
  * 32, BF16 matrix cores, FP32 accumulation) that predicts the fine-grid
  * residual from m and the coarse field.  Trained on the GPU against the exact
  * field.  Used by a simulation (mmx_sim_set_surrogate) for the RK12
@@ -278,9 +279,9 @@ int mmx_surrogate_set_params(mmx_surrogate *sur, const float *host, int count);
 /* Gradients of the last mmx_surrogate_train_step / mmx_surrogate_net. */
 int mmx_surrogate_get_grads(mmx_surrogate *sur, float *host, int count);
 /* One training step on the magnetisation m_dev (device, 3*N floats): exact
- * fine field as the target, forward, MSE loss, backward, Adam with learning
- * rate lr (lr = 0: gradients only).  *loss (may be NULL) = the loss before
- * the update. */
+ * fine field as the target, forward, relative squared error of the residual
+ * (sum d^2 / sum target^2), backward, Adam with learning rate lr (lr = 0:
+ * gradients only).  *loss (may be NULL) = the loss before the update. */
 int mmx_surrogate_train_step(mmx_surrogate *sur, const float *m_dev, float lr, double *loss);
 /* The net alone on a host input x[n1*n2][32] (rounded to bf16): out[n1*n2][16]
  * (FP32); with a target tgt[n1*n2][3*n0] also the loss and the gradients. */

@@ -7,18 +7,20 @@ gradient check and closed-form cases (tests/test_oracle_surrogate.py).
 
 Model (internal z,y,x order, film of n0 layers on an n1 x n2 in-plane grid):
 
-* coarse grid: every axis with more than one cell halved (cell sizes doubled);
-  coarse m = mean of the 2x2x2 (or 2x2 for n0 = 1) fine cells;
+* coarse grid: the film plane halved (in-plane cell sizes doubled), the n0
+  layers kept; coarse m = mean of the 2x2 fine cells of a layer;
 * net input per in-plane pixel (y, x): 32 channels, channel c*n0 + i = m_c at
   layer i, 3*n0 + c*n0 + i = upsampled coarse B_c / (mu0 Ms) at layer i
-  (piecewise constant), the rest 0;
+  (piecewise constant), 6*n0 and 6*n0 + 1 = the position inside the coarse
+  cell (+1 / -1 for odd / even y and x), the rest 0;
 * 4 conv layers 3x3, zero padding: 32 -> 32 -> 32 -> 32 -> 16, ReLU after the
   first three; output channel c*n0 + i = residual (B_fine - B_coarse_up)_c
   / (mu0 Ms) at layer i (channels 3*n0.. unused);
 * bf16 rounding where the GPU rounds: the packed input, the weights, the
   ReLU outputs, the output gradient and the back-propagated gradients; all
   sums in FP32 on the GPU, FP64 here.
-* loss = mean over pixels and the 3*n0 used channels of (out - target)^2.
+* loss = sum (out - target)^2 / sum target^2 over pixels and the 3*n0 used
+  channels (relative squared error of the sample).
 
 Weights: W[l] shape (3, 3, Cin, Cout), out[y, x] = b + sum_{dy,dx} in[y+dy-1,
 x+dx-1] @ W[dy, dx].
@@ -38,21 +40,17 @@ def bf16(x):
 
 
 def coarse_shape(n):
-    return tuple(max(1, v // 2) for v in n)
+    return (n[0], n[1] // 2, n[2] // 2)
 
 
 def coarsen(m, n):
-    """(3, n0, n1, n2) -> (3, nc0, nc1, nc2), block means."""
+    """(3, n0, n1, n2) -> (3, n0, n1/2, n2/2), means of 2x2 in-plane blocks."""
     n0, n1, n2 = n
-    f0 = 2 if n0 > 1 else 1
-    a = m.reshape(3, n0 // f0, f0, n1 // 2, 2, n2 // 2, 2)
-    return a.mean(axis=(2, 4, 6))
+    return m.reshape(3, n0, n1 // 2, 2, n2 // 2, 2).mean(axis=(3, 5))
 
 
 def upsample(bc, n):
-    n0 = n[0]
-    f0 = 2 if n0 > 1 else 1
-    return np.repeat(np.repeat(np.repeat(bc, f0, axis=1), 2, axis=2), 2, axis=3)
+    return np.repeat(np.repeat(bc, 2, axis=2), 2, axis=3)
 
 
 def pack_input(m, bc_up, msat):
@@ -63,6 +61,8 @@ def pack_input(m, bc_up, msat):
         for i in range(n0):
             x[:, :, c * n0 + i] = m[c, i]
             x[:, :, 3 * n0 + c * n0 + i] = bc_up[c, i] / (MU0 * msat)
+    x[:, :, 6 * n0] = np.where(np.arange(n1) % 2 == 1, 1.0, -1.0)[:, None]
+    x[:, :, 6 * n0 + 1] = np.where(np.arange(n2) % 2 == 1, 1.0, -1.0)[None, :]
     return bf16(x)
 
 
@@ -102,13 +102,13 @@ def conv3x3_backward_weight(x, dz):
 
 
 def init_params(seed=0, n0=4):
-    """He-initialised weights (first layer over its 6*n0 used inputs), a small
+    """He-initialised weights (first layer over its 6*n0 + 2 used inputs), a small
     last layer (initial residual ~ 0), zero biases."""
     rng = np.random.default_rng(seed)
     Ws, bs = [], []
     for l in range(4):
         cin, cout = CH[l], CH[l + 1]
-        fan = 9 * (6 * n0 if l == 0 else cin)
+        fan = 9 * (6 * n0 + 2 if l == 0 else cin)
         w = rng.normal(size=(3, 3, cin, cout)) * np.sqrt(2.0 / fan)
         if l == 3:
             w *= 0.1
@@ -136,13 +136,14 @@ def forward(Ws, bs, x0, rnd=bf16):
 
 
 def loss_and_grad_out(out, target, nused, rnd=bf16):
-    """MSE over the nused channels; d loss / d out rounded to bf16."""
+    """Relative squared error over the nused channels; d loss / d out
+    rounded to bf16."""
     rnd = rnd or _ident
-    H, W, _ = out.shape
     d = out[:, :, :nused] - target
-    loss = float((d * d).mean())
+    t2 = float((np.asarray(target, np.float64) ** 2).sum())
+    loss = float((d * d).sum()) / t2
     g = np.zeros_like(out)
-    g[:, :, :nused] = 2.0 * d / (H * W * nused)
+    g[:, :, :nused] = 2.0 * d / t2
     return loss, rnd(g)
 
 

@@ -4,7 +4,7 @@
   of the 4 layers) against the FP64 oracle (oracle/surrogate.py) that rounds
   to bf16 at the same points: the remaining differences are FP32 vs FP64
   accumulation order and the bf16 roundings it flips (tolerances below);
-* the coarse pipeline (block means, coarse exact demag, upsampling) against
+* the coarse pipeline (in-plane block means, coarse exact demag, upsampling) against
   the oracle's coarse convolution, 1e-5;
 * training on exact-solver trajectories lowers the error of the fine field
   below the coarse-only method's (the gain of the network);
@@ -63,7 +63,7 @@ def test_net_forward_loss_gradients_against_oracle(mmx_gpu, n):
     # parameters as the GPU holds them (FP32)
     Ws, bs = mmx.Surrogate.unflatten(sur.get_params())
     x = np.zeros((n[1], n[2], 32))
-    x[:, :, :6 * n0] = rng.normal(size=(n[1], n[2], 6 * n0))
+    x[:, :, :6 * n0 + 2] = rng.normal(size=(n[1], n[2], 6 * n0 + 2))
     tgt = rng.normal(size=(n[1], n[2], 3 * n0)) * 0.3
     out_g, loss_g = sur.net(x, tgt)
     xo = OS.bf16(x)
@@ -76,7 +76,7 @@ def test_net_forward_loss_gradients_against_oracle(mmx_gpu, n):
     assert abs(loss_g - loss_o) < 1e-2 * loss_o
     gW, gb = mmx.Surrogate.unflatten(sur.get_grads())
     for l in range(4):
-        used = slice(None) if l else slice(0, 6 * n0)
+        used = slice(None) if l else slice(0, 6 * n0 + 2)
         ew = _rel(gW[l][:, :, used], dWs[l][:, :, used])
         eb = _rel(gb[l][:3 * n0] if l == 3 else gb[l], dbs[l][:3 * n0] if l == 3 else dbs[l])
         cos = float(np.sum(gW[l] * dWs[l]) / (np.linalg.norm(gW[l]) * np.linalg.norm(dWs[l])))
@@ -97,7 +97,7 @@ def test_coarse_pipeline_against_oracle(mmx_gpu):
     sur.predict_b(dm.data_ptr(), db.data_ptr(), coarse_only=True)
     torch.cuda.synchronize()
     nc = OS.coarse_shape(n)
-    cc = (CELL[0] * 2, CELL[1] * 2, CELL[2] * 2)
+    cc = (CELL[0], CELL[1] * 2, CELL[2] * 2)
     pc = O.pad_size(nc)
     kh, _ = O.kernel_fft(pc, K9=O.kernel_arne(pc, cc))
     mc = OS.coarsen(m.astype(np.float64), n)
@@ -116,32 +116,41 @@ def test_training_beats_coarse_only(mmx_gpu):
     n = (4, 64, 64)
     sim = _sim(mmx, n)
     sur = mmx.Surrogate(sim, seed=3)
-    hist = T.train(sur, sim, nsamples=600, steps_per_sample=2, samples_per_trajectory=50,
-                   lr=2e-3, lr_final=2e-4, alpha=0.2)
-    first = np.mean([l for _, l in hist[:20]])
-    last = np.mean([l for _, l in hist[-20:]])
-    _say("loss %.3e -> %.3e" % (first, last))
-    assert last < 0.5 * first
     plan = mmx.DemagPlan(n, CELL)
-    gains = []
-    for k in range(40, 44):          # textures not used in training
-        m = synth.training_texture(k * 7 + 1, n, CELL)
-        # relax a little, like the states the solver meets
-        sim.setarray("m", m)
+    # validation states: textures not used in training, relaxed a little like
+    # the states the solver meets
+    val = []
+    for k in range(4):
+        sim.setarray("m", synth.training_texture(1001 + 7 * k, n, CELL))
         sim.setv("alpha", 0.2)
         sim.steps(20)
-        dm = torch.from_numpy(sim.getarray("m")).cuda()
-        exact = torch.empty_like(dm)
-        plan.update_b(exact.data_ptr(), dm.data_ptr(), 8e5)
-        bn, bco = torch.empty_like(dm), torch.empty_like(dm)
-        sur.predict_b(dm.data_ptr(), bn.data_ptr())
-        sur.predict_b(dm.data_ptr(), bco.data_ptr(), coarse_only=True)
-        torch.cuda.synchronize()
-        e = exact.cpu().numpy().astype(np.float64)
-        e_net, e_co = _rel(bn.cpu().numpy(), e), _rel(bco.cpu().numpy(), e)
-        _say("texture %d: coarse-only %.3e, with net %.3e" % (k, e_co, e_net))
-        gains.append(e_co / e_net)
-    assert min(gains) > 1.5, gains
+        val.append(sim.getarray("m"))
+
+    def errors():
+        out = []
+        for m in val:
+            dm = torch.from_numpy(m).cuda()
+            exact, bn, bco = torch.empty_like(dm), torch.empty_like(dm), torch.empty_like(dm)
+            plan.update_b(exact.data_ptr(), dm.data_ptr(), 8e5)
+            sur.predict_b(dm.data_ptr(), bn.data_ptr())
+            sur.predict_b(dm.data_ptr(), bco.data_ptr(), coarse_only=True)
+            torch.cuda.synchronize()
+            e = exact.cpu().numpy().astype(np.float64)
+            out.append((_rel(bco.cpu().numpy(), e), _rel(bn.cpu().numpy(), e)))
+        return out
+
+    before = errors()
+    T.train(sur, sim, nsamples=4000, steps_per_sample=2, samples_per_trajectory=50,
+            lr=5e-4, lr_final=2e-4, alpha=0.2)
+    after = errors()
+    for (co, n0_), (_, n1_) in zip(before, after):
+        _say("coarse-only %.3e, untrained net %.3e, trained net %.3e" % (co, n0_, n1_))
+    # the trained net beats the coarse field on every unseen state, by 1.3x on
+    # the geometric mean (4000 pairs; the benchmark trains on 20000)
+    assert all(a[1] < a[0] for a in after), after
+    gain = float(np.exp(np.mean([np.log(a[0] / a[1]) for a in after])))
+    _say("geometric-mean gain %.2f" % gain)
+    assert gain > 1.3, after
 
 
 def test_surrogate_in_rk12_and_fallback(mmx_gpu):
