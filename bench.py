@@ -408,7 +408,12 @@ def run_surrogate(args):
         x.step()
         torch.cuda.synchronize()
         ts = time.perf_counter()
-        x.run(T if T is not None else args.sim_time)
+        T_ = T if T is not None else args.sim_time
+        t_start = x.getv("t")
+        for q in range(1, 11):        # in tenths, with a progress line each
+            x.run(max(0.0, t_start + q * T_ / 10 - x.getv("t")))
+            say("  t %.3e s, %d steps, %.1f s wall" % (x.getv("t"), x.getv("step"),
+                                                       time.perf_counter() - ts))
         torch.cuda.synchronize()
         wall = time.perf_counter() - ts
         out = {"wall_s": wall, "simulated_ns_per_s": x.getv("t") * 1e9 / wall,
@@ -434,7 +439,9 @@ def run_surrogate(args):
     relax = make()
     relax.setarray("m", synth.vortex_m(77, n, cell))
     relax.setv("alpha", 0.5)
-    relax.run(3e-10)
+    for q in range(5):
+        relax.run(4e-11)
+        say("relaxing the vortex: t %.2e, %d steps" % (relax.getv("t"), relax.getv("step")))
     mv = relax.getarray("m")
     bx = tuple(synth.to_internal((5e-3, 0.0, 0.0)))
     gy = {"exact": run(False, m0=mv, bext=bx)}
