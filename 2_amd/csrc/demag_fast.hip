@@ -196,6 +196,31 @@ z_fast(DemagGeom g, int ntx, const float2 *__restrict__ tw, const float *__restr
     const bool ok = kx0 + w < g.h2;
     const long istride = (long)g.p1 * g.sx;
     float2 *base = yb + (long)c * g.n0 * istride + (long)ky * g.sx + kx0 + w;
+    const bool my = ky > g.p1 / 2;
+    const int kyp = my ? g.p1 - ky : ky;
+    const float sy = my ? -1.0f : 1.0f;
+    // the kernel octant points this thread multiplies, loaded before the
+    // spectrum so that their latency hides behind the loads and the forward
+    // transform (when they fit in a few registers)
+    constexpr int NITEM = ((L / 2 + 1) * Z::W + Z::NT - 1) / Z::NT;
+    constexpr int NPF = (NITEM <= 4) ? NITEM : 1;
+    float2 kpf[NPF][3];
+    if constexpr (NITEM <= 4) {
+#pragma unroll
+        for (int r = 0; r < NITEM; r++) {
+            const int item = threadIdx.x + r * Z::NT;
+            const int w2 = item % Z::W, kzp = item / Z::W;
+            if (item < (L / 2 + 1) * Z::W && kx0 + w2 < g.h2) {
+                const long o = ((long)kzp * g.ko1 + kyp) * g.h2 + kx0 + w2;
+                const float2 *kp = reinterpret_cast<const float2 *>(K + 6 * o);
+                kpf[r][0] = kp[0];
+                kpf[r][1] = kp[1];
+                kpf[r][2] = kp[2];
+            } else {
+                kpf[r][0] = kpf[r][1] = kpf[r][2] = float2{0.0f, 0.0f};
+            }
+        }
+    }
     float2 v[Z::E];
 #pragma unroll
     for (int q = 0; q < Z::E; q++) {
@@ -213,13 +238,7 @@ z_fast(DemagGeom g, int ntx, const float2 *__restrict__ tw, const float *__restr
         for (int k2 = 0; k2 < Z::T; k2++)
             sm[(c * L + t * Z::G + gg + Z::E * k2) * Z::W + w] = u[gg][k2];
     __syncthreads();
-    const bool my = ky > g.p1 / 2;
-    const int kyp = my ? g.p1 - ky : ky;
-    const float sy = my ? -1.0f : 1.0f;
-    for (int item = threadIdx.x; item < (L / 2 + 1) * Z::W; item += Z::NT) {
-        const int w2 = item % Z::W, kzp = item / Z::W;
-        if (kx0 + w2 >= g.h2) continue;
-        const Kern6 k = kern_octant(g, K, kzp, kyp, kx0 + w2, 1.0f, sy);
+    auto multiply = [&](const Kern6 &k, int kzp, int w2) {
         kern_apply(k, sm[(0 * L + kzp) * Z::W + w2], sm[(1 * L + kzp) * Z::W + w2],
                    sm[(2 * L + kzp) * Z::W + w2]);
         if (kzp > 0 && kzp < L / 2) {
@@ -229,6 +248,29 @@ z_fast(DemagGeom g, int ntx, const float2 *__restrict__ tw, const float *__restr
             const int kz2 = L - kzp;
             kern_apply(km, sm[(0 * L + kz2) * Z::W + w2], sm[(1 * L + kz2) * Z::W + w2],
                        sm[(2 * L + kz2) * Z::W + w2]);
+        }
+    };
+    if constexpr (NITEM <= 4) {
+#pragma unroll
+        for (int r = 0; r < NITEM; r++) {
+            const int item = threadIdx.x + r * Z::NT;
+            const int w2 = item % Z::W, kzp = item / Z::W;
+            if (item >= (L / 2 + 1) * Z::W || kx0 + w2 >= g.h2) continue;
+            // the layout of kern_octant (XX,YY | ZZ,YZ | XZ,XY), z sign +1
+            Kern6 k;
+            k.xx = kpf[r][0].x;
+            k.yy = kpf[r][0].y;
+            k.zz = kpf[r][1].x;
+            k.yz = sy * kpf[r][1].y;
+            k.xz = kpf[r][2].x;
+            k.xy = sy * kpf[r][2].y;
+            multiply(k, kzp, w2);
+        }
+    } else {
+        for (int item = threadIdx.x; item < (L / 2 + 1) * Z::W; item += Z::NT) {
+            const int w2 = item % Z::W, kzp = item / Z::W;
+            if (kx0 + w2 >= g.h2) continue;
+            multiply(kern_octant(g, K, kzp, kyp, kx0 + w2, 1.0f, sy), kzp, w2);
         }
     }
     __syncthreads();
