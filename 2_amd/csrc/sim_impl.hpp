@@ -27,6 +27,7 @@ struct mmx_sim {
     long step = 0, badsteps = 0, evals = 0;
     bool m_normalized = false;
 
+    int solver = 0; // 0 solver/rk12, 1 solver/heun, 2 solver/euler
     int kernel_type = MMX_KERNEL_ARNE, accuracy = 6;
     mmx_demag_plan *plan = nullptr;
     mmx_reductor *red = nullptr;

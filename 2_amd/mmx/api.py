@@ -3,7 +3,8 @@
 Mirrors the functions a mumax/2 script calls (src/python/mumax2.py, generated
 from src/mumax/engine/api.go): setgridsize, setcellsize, setperiodic, load,
 setv, getv, gets, setarray, getarray, makearray, setmask, run, step, steps,
-run_until_smaller, autotabulate / tabulate.  The reference's API layer swaps
+run_until_smaller, autotabulate / tabulate, save / saveas / autosave / remove,
+outputdir, filenumberformat (output formats in mmx.output).  The reference's API layer swaps
 user XYZ into the engine's internal ZYX order (engine/api.go:43-45,
 frontend/jsonrpc.go:297-312); so does this module, over mmx.Sim.
 
@@ -31,6 +32,8 @@ class Engine:
 
     VECTORS = ("B_ext", "anisU", "j", "<m>")
 
+    UNITS = {"m": "", "H_eff": "A/m", "B": "T", "torque": "/s"}   # modules/*.go AddNewQuant
+
     def __init__(self):
         self.size = None
         self.cell = None
@@ -38,6 +41,12 @@ class Engine:
         self.sim = None
         self._pending = []
         self._tables = []
+        self._autosaves = {}
+        self._handles = 0
+        self.out_dir = ""
+        self.filename_format = "%06d"      # engine.go:84
+        self._last_output_t = None
+        self._output_id = 0
 
     # -- geometry ---------------------------------------------------------
     def setgridsize(self, nx, ny, nz):
@@ -61,8 +70,7 @@ class Engine:
 
     # -- modules / quantities -------------------------------------------
     def load(self, module):
-        if module in ("micromagnetism", "demag", "exchange6", "zeeman", "llg", "solver/rk12",
-                      "maxtorque"):
+        if module in ("micromagnetism", "demag", "exchange6", "zeeman", "llg", "maxtorque"):
             return
         self._need().load(module)
 
@@ -108,10 +116,81 @@ class Engine:
         else:
             s.setmask(name, np.ascontiguousarray(np.asarray(a, np.float32).transpose(2, 1, 0)), 1)
 
+    # -- output (engine.go:98-105,554-582, notify_autosave.go:25-31) -----
+    def outputdir(self, d):
+        self.out_dir = d
+
+    def filenumberformat(self, fmt):
+        self.filename_format = fmt
+
+    def _output_id(self):
+        t = self._need().getv("t")
+        if t != self._last_output_t:
+            self._last_output_t = t
+            self._output_id += 1
+        return self._output_id
+
+    def autofilename(self, quantity, format):
+        name = quantity + (self.filename_format % self._output_id()) + "." + format
+        return (self.out_dir + "/" if self.out_dir else "") + name
+
+    def _field(self, quantity):
+        s = self._need()
+        a = np.asarray(s.getarray(quantity), dtype=np.float64)
+        if quantity == "torque":          # Quant.Buffer applies the multiplier gamma
+            a = a * s.getv("gamma")
+        return a.astype(np.float32)
+
+    def saveas(self, quantity, format, options, filename):
+        """Engine.SaveAs: formats dump, ovf (OVF 2.0), omf (OMF 1.0); options
+        ["Text"] or ["Binary 4"] (the default) for ovf/omf."""
+        from . import output as O
+        s = self._need()
+        data = self._field(quantity)
+        t = s.getv("t")
+        unit = self.UNITS.get(quantity, "")
+        opts = list(options or [])
+        if format == "dump":
+            if opts:
+                raise ValueError("dump output format does not take options")
+            blob = O.dump_bytes(data, s.cell, t, quantity, unit)
+        elif format in ("ovf", "omf"):
+            if len(opts) > 1:
+                raise ValueError("Illegal %s options: %s" % (format.upper(), opts))
+            df = opts[0] if opts else "Binary 4"
+            blob = (O.ovf2_text if format == "ovf" else O.omf_text)(data, s.cell, t, quantity,
+                                                                     unit, df)
+        else:
+            raise ValueError("Unknown output format: %s. Options are: dump ovf omf" % format)
+        with open(filename, "wb") as f:
+            f.write(blob)
+        return filename
+
+    def save(self, quantity, format, options=()):
+        return self.saveas(quantity, format, options, self.autofilename(quantity, format))
+
+    def autosave(self, quantity, format, options=(), period=0.0):
+        self._handles += 1
+        self._autosaves[self._handles] = dict(q=quantity, f=format, o=list(options or []),
+                                              period=float(period), start=self._need().getv("t"),
+                                              count=0)
+        return self._handles
+
+    def remove(self, handle):
+        if handle in self._autosaves:
+            del self._autosaves[handle]
+        else:
+            self._tables = [t for t in self._tables if getattr(t, "handle", None) != handle]
+
     # -- time stepping --------------------------------------------------
     def _notify(self):
         for tab in self._tables:
             tab.notify(self)
+        for a in self._autosaves.values():
+            t = self._need().getv("t") - a["start"]
+            if t - a["count"] * a["period"] >= a["period"]:
+                self.save(a["q"], a["f"], a["o"])
+                a["count"] += 1
 
     def step(self):
         self._need().step()
@@ -124,7 +203,7 @@ class Engine:
     def run(self, duration):
         s = self._need()
         start = s.getv("t")
-        if not self._tables:
+        if not self._tables and not self._autosaves:
             s.run(duration)
             return
         while s.getv("t") < start + duration:
@@ -136,30 +215,64 @@ class Engine:
             self.step()
 
     def autotabulate(self, quants, filename=None, period=0.0):
-        """notify_tabulate.go:23-28: a row at the first step past each period."""
-        tab = AutoTable(quants, period)
+        """notify_tabulate.go:23-28: a row at the first step past each period;
+        with a file name the rows are also appended to that table file
+        (table.go:26-71)."""
+        tab = AutoTable(quants, period, filename)
+        self._handles += 1
+        tab.handle = self._handles
         self._tables.append(tab)
         return tab
 
-    def tabulate(self, quants):
-        return [self.getv(q) for q in quants]
+    def tabulate(self, quants, filename=None):
+        """Engine.Tabulate: the values (user order); with a file name, one
+        row appended to that table file."""
+        vals = [self.getv(q) for q in quants]
+        if filename:
+            _append_row(filename, self, quants, vals)
+        return vals
+
+
+_UNIT_OF = {"t": "s", "dt": "s", "<m>": "", "maxtorque": "/s", "B_ext": "T", "j": "A/m2",
+            "E": "J", "E_demag": "J", "E_ex": "J", "E_zeeman": "J", "E_anis": "J",
+            "Msat": "A/m", "Aex": "J/m", "alpha": "", "step": "", "m_error": "",
+            "m_peakerror": "", "badsteps": ""}
+
+
+def _append_row(filename, e, quants, vals):
+    """table.go:26-49: the header at the first row, then values + tabs."""
+    from . import output as O
+    import os
+    fresh = not os.path.exists(filename) or os.path.getsize(filename) == 0
+    with open(filename, "a") as f:
+        if fresh:
+            spec = [(q, len(v) if isinstance(v, (list, tuple)) else 1, _UNIT_OF.get(q, ""))
+                    for q, v in zip(quants, vals)]
+            f.write(O.table_header(spec))
+        flat = []
+        for v in vals:
+            flat.extend(v if isinstance(v, (list, tuple)) else [v])
+        f.write(O.table_row(flat))
 
 
 class AutoTable:
-    def __init__(self, quants, period):
+    def __init__(self, quants, period, filename=None):
         self.quants = list(quants)
         self.period = float(period)
+        self.filename = filename
         self.count = 0
         self.rows = []
 
     def notify(self, e):
         t = e.getv("t")
         if t - self.count * self.period >= self.period:
+            vals = [e.getv(q) for q in self.quants]
             row = []
-            for q in self.quants:
-                v = e.getv(q)
+            for v in vals:
                 row.extend(v if isinstance(v, (list, tuple)) else [v])
             self.rows.append(row)
+            if self.filename:
+                _append_row(self.filename, e, self.quants, vals)
             self.count += 1
 
 
@@ -178,5 +291,6 @@ def reset():
 
 for _name in ("setgridsize", "setcellsize", "setperiodic", "load", "setv", "getv", "gets",
               "setarray", "getarray", "setmask", "step", "steps", "run", "run_until_smaller",
-              "autotabulate", "tabulate"):
+              "autotabulate", "tabulate", "save", "saveas", "autosave", "remove", "outputdir",
+              "filenumberformat"):
     globals()[_name] = (lambda n: lambda *a, **k: getattr(_default, n)(*a, **k))(_name)

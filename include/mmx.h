@@ -210,8 +210,10 @@ typedef struct mmx_sim mmx_sim;
 int mmx_sim_create(mmx_sim **sim, int n0, int n1, int n2, double cell0,
                    double cell1, double cell2, int pbc0, int pbc1, int pbc2);
 int mmx_sim_destroy(mmx_sim *sim);
-/* Engine.LoadModule: "anisotropy/uniaxial", "zhang-li" ("micromagnetism",
- * "demag", "exchange6", "zeeman", "llg", "solver/rk12", "maxtorque" are
+/* Engine.LoadModule: "anisotropy/uniaxial", "zhang-li", and the solvers
+ * "solver/rk12" (default, adaptive), "solver/heun" and "solver/euler" (fixed
+ * dt; solve_heun.go:27-79, solve_euler.go:22-50), which replace each other
+ * ("micromagnetism", "demag", "exchange6", "zeeman", "llg", "maxtorque" are
  * already loaded). */
 int mmx_sim_load(mmx_sim *sim, const char *module);
 /* Demag kernel choice (default MMX_KERNEL_ARNE, accuracy 6); a change
@@ -253,6 +255,12 @@ int mmx_sim_time_passes(mmx_sim *sim, int reps, double *ms, int *npass);
 /* Same-dt replay: one predictor/corrector try with the given dt, no step
  * control (no reference counterpart; used to compare m(t) step by step). */
 int mmx_sim_try_step(mmx_sim *sim, double dt, double *err);
+
+/* CRC-64/ISO as Go's hash/crc64 with crc64.ISO computes it, continuing
+ * from crc (0 to start): the checksum of the #dump002 output format
+ * (src/mumax/dump/writer.go:12,30-37,60-67).  Host memory, no GPU. */
+unsigned long long mmx_crc64_iso(const void *data, unsigned long long n,
+                                 unsigned long long crc);
 
 /* ------------------------------------------------------------------------
  * Learned demag correction (no reference counterpart).  For a film of

@@ -136,3 +136,38 @@ def test_sp5_against_reference_trace(api):
     fin = rows[-1]
     assert abs(fin[1] - ref[-1, 1]) < 0.02 and abs(fin[2] - ref[-1, 2]) < 0.02
     assert dxy[:, 2].max() < 1e-3
+
+
+def test_save_autosave_and_table_files(api, tmp_path):
+    """save / autosave / autotabulate through the user-order API
+    (engine/api.go:514-564): files in the reference's formats whose contents
+    read back to the engine's arrays and values."""
+    from mmx import output as O
+    e = api.reset()
+    e.setgridsize(32, 16, 2)
+    e.setcellsize(4e-9, 4e-9, 3e-9)
+    e.load("micromagnetism")
+    e.setv("Msat", 8e5)
+    e.setv("Aex", 1.3e-11)
+    e.setv("alpha", 0.5)
+    e.setarray("m", [[[[1]]], [[[0.2]]], [[[0.1]]]])
+    e.outputdir(str(tmp_path))
+    f1 = e.save("m", "dump")
+    f2 = e.save("H_eff", "ovf", ["Text"])
+    assert f1.endswith("m000001.dump") and f2.endswith("H_eff000001.ovf")
+    hdr, data, _ = O.read_dump(open(f1, "rb").read())
+    m_int = e.sim.getarray("m")
+    assert hdr["meshsize"] == (2, 16, 32) and np.array_equal(data, m_int)
+    h = e.autosave("m", "ovf", [], 1e-12)
+    table = str(tmp_path / "table.txt")
+    e.autotabulate(["t", "<m>"], table, 1e-12)
+    e.run(5.5e-12)
+    outs = sorted(p.name for p in tmp_path.iterdir() if p.name.startswith("m0") and p.suffix == ".ovf")
+    assert len(outs) == 5, outs
+    vals = O.read_ovf_binary(open(tmp_path / outs[-1], "rb").read())
+    assert vals.shape == (2 * 16 * 32, 3)
+    rows = open(table).read().splitlines()
+    assert rows[0] == "#t (s)\t<m>_x ()\t<m>_y ()\t<m>_z ()\t" and len(rows) == 6
+    e.remove(h)
+    e.run(2e-12)
+    assert len([p for p in tmp_path.iterdir() if p.name.startswith("m0") and p.suffix == ".ovf"]) == 5
