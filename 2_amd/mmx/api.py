@@ -46,7 +46,7 @@ class Engine:
         self.out_dir = ""
         self.filename_format = "%06d"      # engine.go:84
         self._last_output_t = None
-        self._output_id = 0
+        self._outid = 0
 
     # -- geometry ---------------------------------------------------------
     def setgridsize(self, nx, ny, nz):
@@ -127,8 +127,8 @@ class Engine:
         t = self._need().getv("t")
         if t != self._last_output_t:
             self._last_output_t = t
-            self._output_id += 1
-        return self._output_id
+            self._outid += 1
+        return self._outid
 
     def autofilename(self, quantity, format):
         name = quantity + (self.filename_format % self._output_id()) + "." + format

@@ -362,7 +362,8 @@ def test_fixed_step_solvers(torch_gpu, solver):
         if solver == "solver/heun":
             prob.try_step(mo, dt)
         else:
-            mn = prob.normalized(mo)
+            # the normUpdater before each step's first stage (every step)
+            mn = O.normalize(mo)
             tau = prob.torque(mn, prob.heff(mn))
             mo = mn + (O.GAMMA0 * dt) * tau
     assert abs(sim.getv("t") - 12 * dt) < 1e-6 * dt and sim.getv("step") == 12
