@@ -117,6 +117,24 @@ def summarize_profiles(kt_db, fetch_db=None, write_db=None):
     return out
 
 
+def measured_traffic(pass_name, summary="profiles/demag_passes_bulk_256x256x64.md"):
+    """HBM bytes per launch of a pass kernel from the committed counter summary
+    (rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs, scaled by the factors
+    that make the 1 GiB copy probe read and write 1 GiB), or None."""
+    path = os.path.join(ROOT, summary)
+    if not os.path.exists(path):
+        return None
+    key = "`mmx::%s_fast<" % pass_name
+    for line in open(path):
+        if line.startswith("| " + key) and line.count("|") == 5:
+            cells = [c.strip() for c in line.strip().strip("|").split("|")]
+            try:
+                return float(cells[3]) * 1e6
+            except ValueError:
+                return None
+    return None
+
+
 def _time_evals(sim, steps, warmup):
     import torch
     for _ in range(warmup):
@@ -631,7 +649,10 @@ def main():
                    "steps_are": "H_eff evaluations",
                    "exact_path_only": True},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": measured_traffic(dom),
+                     "traffic_source": "profiles/demag_passes_bulk_256x256x64.md (FETCH_SIZE x2 + "
+                                       "WRITE_SIZE x1 per dispatch)"},
         "cpu_baseline": cpu,
         "passes_ms": passes,
         "pass_bytes": nbytes,
