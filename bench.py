@@ -125,8 +125,11 @@ def measured_traffic(pass_name, summary="profiles/demag_passes_bulk_256x256x64.m
     if not os.path.exists(path):
         return None
     key = "`mmx::%s_fast<" % pass_name
+    in_table = False
     for line in open(path):
-        if line.startswith("| " + key) and line.count("|") == 5:
+        if "scaled read+write" in line:
+            in_table = True
+        elif in_table and line.startswith("| " + key) and line.count("|") == 5:
             cells = [c.strip() for c in line.strip().strip("|").split("|")]
             try:
                 return float(cells[3]) * 1e6
