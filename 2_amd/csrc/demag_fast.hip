@@ -18,7 +18,7 @@ template <int L> struct ColTile {
     static constexpr int E = Split<L>::E;
     static constexpr int G = E / T;
     static constexpr int W = (L == 2048) ? 4 : (L == 1024 ? 8 : 256 / T);
-    static constexpr int S = T * W + 8;
+    static constexpr int S = lds_pitch(T, W, G);
     static constexpr int NT = T * W;                     // threads
     static constexpr int LDS = (E * S + L) * 8;           // transpose + twiddles
 };
@@ -41,13 +41,13 @@ template <int LX> struct RowTile {
 // Thread (w, t) of a column tile: w = tid % W (column), t = tid / W.
 // Transpose buffer index of Y[t][k1] of column w: k1*S + t*W + w.
 
-template <int L, int W, int S>
+template <int L, int W, int S, bool HALFIN = false>
 __device__ __forceinline__ void col_forward(float2 (&v)[Split<L>::E],
                                             float2 (&u)[Split<L>::E / Split<L>::T][Split<L>::T],
                                             int w, int t, float2 *sbuf, const float2 *stw)
 {
     constexpr int T = Split<L>::T, E = Split<L>::E, G = E / T;
-    fs_first<L, -1>(v, t, stw);
+    fs_first<L, -1, HALFIN>(v, t, stw);
 #pragma unroll
     for (int k1 = 0; k1 < E; k1++) sbuf[k1 * S + t * W + w] = v[k1];
     __syncthreads();
@@ -159,11 +159,36 @@ template <int L> struct ZTile {
     static constexpr int G = E / T;
     // 8 columns for L = 128 (measured: 141 us vs 145 us with 16 at 256x256x64)
     static constexpr int W = (L == 2048) ? 2 : (L == 128 ? 8 : 128 / T);
-    static constexpr int S = T * W + 8;
+    static constexpr int S = lds_pitch(T, W, G);
     static constexpr int NG = T * W;          // threads per component group
     static constexpr int NT = 3 * NG;
     static constexpr int LDS = (3 * E * S + L) * 8;
+    // spectrum rows (one kz, W columns) are exchanged by lanes (t, w) at rows
+    // t*G + g: with G = 2 only every other row's banks would be used, so bit
+    // 0 of the row is flipped by the row bit that advances one bank span
+    static constexpr int SWZ = (G == 2 && W < 32) ? __builtin_ctz(32 / W) : -1;
+    static_assert(NG % 64 == 0, "component groups must be whole waves");
+    static_assert(NT % W == 0, "");
 };
+
+template <int L> __device__ __forceinline__ int zrow(int kz)
+{
+    if constexpr (ZTile<L>::SWZ >= 0)
+        return kz ^ ((kz >> ZTile<L>::SWZ) & 1);
+    else
+        return kz;
+}
+
+// *(T *)(base + off) with a 32-bit byte offset: with a wave-uniform base this
+// is one global access with a scalar base and a vector offset.
+template <typename T> __device__ __forceinline__ T ld_off(const void *base, unsigned off)
+{
+    return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + off);
+}
+template <typename T> __device__ __forceinline__ void st_off(void *base, unsigned off, T v)
+{
+    *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + off) = v;
+}
 
 // ky visiting order 0, p/2, 1, p-1, 2, p-2, ...: tiles of ky and p - ky
 // (which read the same kernel octant rows) are neighbours
@@ -179,24 +204,32 @@ __device__ __forceinline__ int mirror_order(int i, int p)
     return (i & 1) ? p - u : u;
 }
 
-template <int L>
+// FULL: periodic z (n0 == L, every plane holds data); otherwise n0 == L/2
+// and planes n0 .. L-1 are zero padding (not read, not written).  Byte
+// offsets from the wave-uniform bases stay below 4 GiB (fast_z_addressable).
+template <int L, bool FULL>
 __global__ void __launch_bounds__(ZTile<L>::NT)
 z_fast(DemagGeom g, int ntx, const float2 *__restrict__ tw, const float *__restrict__ K,
        float2 *__restrict__ yb)
 {
     using Z = ZTile<L>;
+    constexpr int NQ = FULL ? Z::E : Z::E / 2; // planes t + T q held, q < NQ
     extern __shared__ float2 sm[];
     float2 *stw = sm + 3 * Z::E * Z::S;
     const int tile = xcd_tile(blockIdx.x, gridDim.x);
     const int tx = tile % ntx, ky = mirror_order(tile / ntx, g.p1);
     const int kx0 = tx * Z::W;
-    const int c = threadIdx.x / Z::NG, lt = threadIdx.x % Z::NG;
+    const int c = __builtin_amdgcn_readfirstlane(threadIdx.x / Z::NG);
+    const int lt = threadIdx.x % Z::NG;
     const int w = lt % Z::W, t = lt / Z::W;
     float2 *sbuf = sm + c * Z::E * Z::S;
     for (int q = threadIdx.x; q < L; q += Z::NT) stw[q] = tw[q];
     const bool ok = kx0 + w < g.h2;
-    const long istride = (long)g.p1 * g.sx;
-    float2 *base = yb + (long)c * g.n0 * istride + (long)ky * g.sx + kx0 + w;
+    // columns past h2 load column h2 - 1 (in bounds) and store nothing
+    const int col = ok ? kx0 + w : g.h2 - 1;
+    const size_t istride = (size_t)g.p1 * g.sx;
+    float2 *cbase = yb + (size_t)c * g.n0 * istride + (size_t)ky * g.sx;
+    const unsigned voff = (unsigned)(((size_t)t * istride + col) * sizeof(float2));
     const bool my = ky > g.p1 / 2;
     const int kyp = my ? g.p1 - ky : ky;
     const float sy = my ? -1.0f : 1.0f;
@@ -206,17 +239,19 @@ z_fast(DemagGeom g, int ntx, const float2 *__restrict__ tw, const float *__restr
     constexpr int NITEM = ((L / 2 + 1) * Z::W + Z::NT - 1) / Z::NT;
     constexpr int NPF = (NITEM <= 4) ? NITEM : 1;
     float2 kpf[NPF][3];
+    const float *kbase = K + 6 * ((size_t)kyp * g.h2 + kx0);
+    const size_t kzs = (size_t)g.ko1 * g.h2;   // octant points per kz
     if constexpr (NITEM <= 4) {
+        const int w2 = threadIdx.x % Z::W;
+        const unsigned koff = (unsigned)(((threadIdx.x / Z::W) * kzs + w2) * 6 * sizeof(float));
 #pragma unroll
         for (int r = 0; r < NITEM; r++) {
             const int item = threadIdx.x + r * Z::NT;
-            const int w2 = item % Z::W, kzp = item / Z::W;
+            const float *kb = kbase + 6 * (size_t)(r * (Z::NT / Z::W)) * kzs;
             if (item < (L / 2 + 1) * Z::W && kx0 + w2 < g.h2) {
-                const long o = ((long)kzp * g.ko1 + kyp) * g.h2 + kx0 + w2;
-                const float2 *kp = reinterpret_cast<const float2 *>(K + 6 * o);
-                kpf[r][0] = kp[0];
-                kpf[r][1] = kp[1];
-                kpf[r][2] = kp[2];
+                kpf[r][0] = ld_off<float2>(kb, koff);
+                kpf[r][1] = ld_off<float2>(kb, koff + 8);
+                kpf[r][2] = ld_off<float2>(kb, koff + 16);
             } else {
                 kpf[r][0] = kpf[r][1] = kpf[r][2] = float2{0.0f, 0.0f};
             }
@@ -224,31 +259,29 @@ z_fast(DemagGeom g, int ntx, const float2 *__restrict__ tw, const float *__restr
     }
     float2 v[Z::E];
 #pragma unroll
-    for (int q = 0; q < Z::E; q++) {
-        const int i = t + Z::T * q;
-        v[q] = (ok && i < g.n0) ? base[i * istride] : float2{0.0f, 0.0f};
-    }
+    for (int q = 0; q < NQ; q++) v[q] = ld_off<float2>(cbase + (size_t)q * Z::T * istride, voff);
     __syncthreads();
     float2 u[Z::G][Z::T];
-    col_forward<L, Z::W, Z::S>(v, u, w, t, sbuf, stw);
+    col_forward<L, Z::W, Z::S, !FULL>(v, u, w, t, sbuf, stw);
 
-    // spectrum of all three components through LDS: spec[c][kz][w]
+    // spectrum of all three components through LDS: spec[c][zrow(kz)][w]
 #pragma unroll
     for (int gg = 0; gg < Z::G; gg++)
 #pragma unroll
         for (int k2 = 0; k2 < Z::T; k2++)
-            sm[(c * L + t * Z::G + gg + Z::E * k2) * Z::W + w] = u[gg][k2];
+            sm[(c * L + zrow<L>(t * Z::G + gg + Z::E * k2)) * Z::W + w] = u[gg][k2];
     __syncthreads();
     auto multiply = [&](const Kern6 &k, int kzp, int w2) {
-        kern_apply(k, sm[(0 * L + kzp) * Z::W + w2], sm[(1 * L + kzp) * Z::W + w2],
-                   sm[(2 * L + kzp) * Z::W + w2]);
+        const int r1 = zrow<L>(kzp);
+        kern_apply(k, sm[(0 * L + r1) * Z::W + w2], sm[(1 * L + r1) * Z::W + w2],
+                   sm[(2 * L + r1) * Z::W + w2]);
         if (kzp > 0 && kzp < L / 2) {
             Kern6 km = k; // K at p0 - kz: odd in z for XZ and XY
             km.xz = -km.xz;
             km.xy = -km.xy;
-            const int kz2 = L - kzp;
-            kern_apply(km, sm[(0 * L + kz2) * Z::W + w2], sm[(1 * L + kz2) * Z::W + w2],
-                       sm[(2 * L + kz2) * Z::W + w2]);
+            const int r2 = zrow<L>(L - kzp);
+            kern_apply(km, sm[(0 * L + r2) * Z::W + w2], sm[(1 * L + r2) * Z::W + w2],
+                       sm[(2 * L + r2) * Z::W + w2]);
         }
     };
     if constexpr (NITEM <= 4) {
@@ -279,15 +312,12 @@ z_fast(DemagGeom g, int ntx, const float2 *__restrict__ tw, const float *__restr
     for (int gg = 0; gg < Z::G; gg++)
 #pragma unroll
         for (int k2 = 0; k2 < Z::T; k2++)
-            u[gg][k2] = sm[(c * L + t * Z::G + gg + Z::E * k2) * Z::W + w];
+            u[gg][k2] = sm[(c * L + zrow<L>(t * Z::G + gg + Z::E * k2)) * Z::W + w];
     __syncthreads();
     col_inverse<L, Z::W, Z::S>(u, v, w, t, sbuf, stw);
     if (ok) {
 #pragma unroll
-        for (int q = 0; q < Z::E; q++) {
-            const int i = t + Z::T * q;
-            if (i < g.n0) base[i * istride] = v[q];
-        }
+        for (int q = 0; q < NQ; q++) st_off<float2>(cbase + (size_t)q * Z::T * istride, voff, v[q]);
     }
 }
 
@@ -727,17 +757,39 @@ int fast_yi(const DemagPlanImpl *P, hipStream_t st)
     return MMX_OK;
 }
 
+// The z pass addresses the spectrum and the kernel octant with 32-bit byte
+// offsets from wave-uniform bases: the largest lane offsets are T planes of
+// the spectrum and 3 T kz-planes of the octant.
+bool fast_z_addressable(const DemagGeom &g)
+{
+    int T = 0;
+    switch (g.p0) {
+#define CASE(L)                                                                              \
+    case L: T = ZTile<L>::T; break;
+        MMX_FAST_LENGTHS(CASE)
+#undef CASE
+    default: return false;
+    }
+    const double lim = 4294967295.0;
+    const double spec = ((double)T * g.p1 + 1.0) * g.sx * sizeof(float2);
+    const double kern = ((double)3 * T * g.ko1 + 1.0) * g.h2 * 6 * sizeof(float);
+    return spec < lim && kern < lim && (g.n0 == g.p0 || 2 * g.n0 == g.p0);
+}
+
 int fast_z(const DemagPlanImpl *P, hipStream_t st)
 {
     const DemagGeom &g = P->g;
+    MMX_REQUIRE(fast_z_addressable(g), MMX_ERR_SIZE, "fast_z: unsupported geometry");
+    const bool full = g.n0 == g.p0;
     switch (g.p0) {
 #define CASE(L)                                                                              \
     case L: {                                                                                \
         using Z = ZTile<L>;                                                                  \
         const int ntx = div_up(g.h2, Z::W);                                                  \
-        allow_lds_fast(z_fast<L>, Z::LDS);                                                   \
-        z_fast<L><<<dim3(ntx * g.p1), dim3(Z::NT), Z::LDS, st>>>(g, ntx, P->fz.tw, P->kern,  \
-                                                                 P->ybuf);                   \
+        auto kern = full ? z_fast<L, true> : z_fast<L, false>;                               \
+        allow_lds_fast(kern, Z::LDS);                                                        \
+        kern<<<dim3(ntx * g.p1), dim3(Z::NT), Z::LDS, st>>>(g, ntx, P->fz.tw, P->kern,       \
+                                                            P->ybuf);                        \
         break;                                                                               \
     }
         MMX_FAST_LENGTHS(CASE)

@@ -85,7 +85,7 @@ static int plan_init(DemagPlanImpl *P, int n0, int n1, int n2, const double cell
     P->accuracy = accuracy;
     P->fast_x = fast_length(g.Lx);
     P->fast_y = fast_length(g.p1);
-    P->fast_z = g.p0 > 1 && fast_length(g.p0);
+    P->fast_z = g.p0 > 1 && fast_length(g.p0) && fast_z_addressable(g);
 
     // FFT descriptors
     LineFFT<float> *fs[3] = {&P->fx, &P->fy, &P->fz};

@@ -138,6 +138,33 @@ template <int N, int DIR> __device__ __forceinline__ void fft_reg(float2 (&v)[N]
     }
 }
 
+// In-register FFT of N points whose upper half v[N/2 .. N-1] is zero (those
+// registers are not read): the first radix-2 stage reduces to copies.
+template <int N, int DIR> __device__ __forceinline__ void fft_reg_halfin(float2 (&v)[N])
+{
+    if constexpr (N <= 8) {
+#pragma unroll
+        for (int i = N / 2; i < N; i++) v[i] = float2{0.0f, 0.0f};
+        fft_reg<N, DIR>(v);
+    } else {
+        float2 tmp[N];
+#pragma unroll
+        for (int j = 0; j < N / 2; j++) {
+            tmp[2 * j] = v[j];
+            tmp[2 * j + 1] = v[j];
+        }
+        bool in_b = true;
+        R2Stages<N, DIR, 2>::run(tmp, v, in_b);
+        // the remaining log2(N) - 1 stages alternate v, tmp, v, ...: the result
+        // is in tmp when log2 N is odd
+        constexpr int lg = __builtin_ctz(N);
+        if constexpr (lg & 1) {
+#pragma unroll
+            for (int i = 0; i < N; i++) v[i] = tmp[i];
+        }
+    }
+}
+
 // Calls f(std::integral_constant<int, I>) for I = 0 .. N-1 (compile-time loop).
 template <typename F, int... I>
 __device__ __forceinline__ void static_for_impl(F &&f, std::integer_sequence<int, I...>)
@@ -169,6 +196,17 @@ inline bool fast_length_impl(int L)
            L == 1024 || L == 2048;
 }
 
+// LDS row pitch (in float2) of a column-tile transpose: rows of T*W points,
+// read back by lanes (t, w) at row t*G + g.  A pitch with G*S = W (mod 32)
+// puts the lanes' 64-bit words on distinct bank pairs (64 lanes cover the
+// 64 four-byte banks twice, each half-wave once).
+constexpr int lds_pitch(int T, int W, int G)
+{
+    for (int pad = 0; pad < 32; pad++)
+        if (((G * (T * W + pad) - W) % 32 + 32) % 32 == 0) return T * W + pad;
+    return T * W + 1;
+}
+
 // Blocks are dealt to the 8 XCDs round-robin; this maps a block id to a tile
 // id so that each XCD walks one contiguous range of tiles (its own L2).
 __device__ __forceinline__ int xcd_tile(int bid, int nb)
@@ -178,12 +216,16 @@ __device__ __forceinline__ int xcd_tile(int bid, int nb)
 }
 
 // Forward step 1-2 of the four-step: E-point FFT, then v[k1] *= W_L^(t k1)
-// (twiddle table tw[L] in LDS, forward sign).
-template <int L, int DIR>
+// (twiddle table tw[L] in LDS, forward sign).  HALFIN: the line's upper half
+// (v[E/2 ..]) is zero and not read.
+template <int L, int DIR, bool HALFIN = false>
 __device__ __forceinline__ void fs_first(float2 (&v)[Split<L>::E], int t, const float2 *stw)
 {
     constexpr int E = Split<L>::E;
-    fft_reg<E, DIR>(v);
+    if constexpr (HALFIN)
+        fft_reg_halfin<E, DIR>(v);
+    else
+        fft_reg<E, DIR>(v);
 #pragma unroll
     for (int k1 = 1; k1 < E; k1++) {
         const float2 w = stw[t * k1];

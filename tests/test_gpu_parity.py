@@ -87,7 +87,14 @@ CONV_CASES = [((1, 32, 128), (3e-9, 3.90625e-9, 3.90625e-9), (0, 0, 0)),
               # the longest transforms: y 2048 in 3-D, x half-lengths 1024 and 2048
               ((2, 1024, 24), (2e-9, 2e-9, 2e-9), (0, 0, 0)),
               ((2, 12, 1024), (2e-9, 2e-9, 2e-9), (0, 0, 0)),
-              ((1, 8, 2048), (3e-9, 4e-9, 4e-9), (0, 0, 0))]
+              ((1, 8, 2048), (3e-9, 4e-9, 4e-9), (0, 0, 0)),
+              # z transforms of length 128, 512, 2048 (zero-padded half), and
+              # periodic z (every plane holds data) at lengths 8 and 64
+              ((64, 8, 16), (3e-9, 4e-9, 4e-9), (0, 0, 0)),
+              ((256, 4, 8), (2e-9, 4e-9, 4e-9), (0, 0, 0)),
+              ((1024, 2, 4), (2e-9, 4e-9, 4e-9), (0, 0, 0)),
+              ((8, 16, 16), (3e-9, 4e-9, 4e-9), (1, 0, 0)),
+              ((64, 8, 8), (3e-9, 3e-9, 3e-9), (1, 0, 0))]
 
 
 @pytest.mark.parametrize("n,cell,pbc", CONV_CASES)
