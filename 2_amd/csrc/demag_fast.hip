@@ -83,13 +83,26 @@ __device__ __forceinline__ void col_inverse(float2 (&u)[Split<L>::E / Split<L>::
     __syncthreads();
 }
 
-// yf: FFT along y of the n0 nonzero planes (rows j >= n1 are zeros)
-template <int L>
+// *(T *)(base + off) with a 32-bit byte offset: with a wave-uniform base this
+// is one global access with a scalar base and a vector offset.
+template <typename T> __device__ __forceinline__ T ld_off(const void *base, unsigned off)
+{
+    return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + off);
+}
+template <typename T> __device__ __forceinline__ void st_off(void *base, unsigned off, T v)
+{
+    *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + off) = v;
+}
+
+// yf: FFT along y of the n0 nonzero planes.  FULL: periodic y (n1 == L);
+// otherwise n1 == L/2 and rows n1 .. L-1 are zero padding (not read).
+template <int L, bool FULL>
 __global__ void __launch_bounds__(ColTile<L>::NT)
 yf_fast(DemagGeom g, int ntx, const float2 *__restrict__ tw, const float2 *__restrict__ xb,
         float2 *__restrict__ yb)
 {
     using C = ColTile<L>;
+    constexpr int NQ = FULL ? C::E : C::E / 2; // rows t + T q held, q < NQ
     extern __shared__ float2 sm[];
     float2 *sbuf = sm, *stw = sm + C::E * C::S;
     const int tile = xcd_tile(blockIdx.x, gridDim.x);
@@ -98,31 +111,34 @@ yf_fast(DemagGeom g, int ntx, const float2 *__restrict__ tw, const float2 *__res
     const int w = threadIdx.x % C::W, t = threadIdx.x / C::W;
     for (int q = threadIdx.x; q < L; q += C::NT) stw[q] = tw[q];
     const bool ok = kx0 + w < g.h2;
-    const float2 *src = xb + ((long)c * g.nrows + (long)i * g.n1) * g.sx + kx0 + w;
+    // columns past h2 load column h2 - 1 (in bounds) and store nothing
+    const int col = ok ? kx0 + w : g.h2 - 1;
+    const float2 *src = xb + ((size_t)c * g.nrows + (size_t)i * g.n1) * g.sx;
+    const unsigned roff = (unsigned)(((size_t)t * g.sx + col) * sizeof(float2));
     float2 v[C::E];
 #pragma unroll
-    for (int q = 0; q < C::E; q++) {
-        const int j = t + C::T * q;
-        v[q] = (ok && j < g.n1) ? src[(long)j * g.sx] : float2{0.0f, 0.0f};
-    }
+    for (int q = 0; q < NQ; q++) v[q] = ld_off<float2>(src + (size_t)q * C::T * g.sx, roff);
     __syncthreads();
     float2 u[C::G][C::T];
-    col_forward<L, C::W, C::S>(v, u, w, t, sbuf, stw);
+    col_forward<L, C::W, C::S, !FULL>(v, u, w, t, sbuf, stw);
     if (!ok) return;
-    float2 *dst = yb + ((long)c * g.n0 + i) * (long)g.p1 * g.sx + kx0 + w;
+    float2 *dst = yb + ((size_t)c * g.n0 + i) * (size_t)g.p1 * g.sx;
+    const unsigned woff = (unsigned)(((size_t)t * C::G * g.sx + col) * sizeof(float2));
 #pragma unroll
     for (int gg = 0; gg < C::G; gg++)
 #pragma unroll
-        for (int k2 = 0; k2 < C::T; k2++) dst[(long)(t * C::G + gg + C::E * k2) * g.sx] = u[gg][k2];
+        for (int k2 = 0; k2 < C::T; k2++)
+            st_off<float2>(dst + (size_t)(gg + C::E * k2) * g.sx, woff, u[gg][k2]);
 }
 
-// yi: inverse FFT along y, keep j < n1
-template <int L>
+// yi: inverse FFT along y, keep j < n1 (FULL: periodic y, n1 == L)
+template <int L, bool FULL>
 __global__ void __launch_bounds__(ColTile<L>::NT)
 yi_fast(DemagGeom g, int ntx, const float2 *__restrict__ tw, const float2 *__restrict__ yb,
         float2 *__restrict__ xb)
 {
     using C = ColTile<L>;
+    constexpr int NQ = FULL ? C::E : C::E / 2;
     extern __shared__ float2 sm[];
     float2 *sbuf = sm, *stw = sm + C::E * C::S;
     const int tile = xcd_tile(blockIdx.x, gridDim.x);
@@ -131,23 +147,23 @@ yi_fast(DemagGeom g, int ntx, const float2 *__restrict__ tw, const float2 *__res
     const int w = threadIdx.x % C::W, t = threadIdx.x / C::W;
     for (int q = threadIdx.x; q < L; q += C::NT) stw[q] = tw[q];
     const bool ok = kx0 + w < g.h2;
-    const float2 *src = yb + ((long)c * g.n0 + i) * (long)g.p1 * g.sx + kx0 + w;
+    const int col = ok ? kx0 + w : g.h2 - 1;
+    const float2 *src = yb + ((size_t)c * g.n0 + i) * (size_t)g.p1 * g.sx;
+    const unsigned roff = (unsigned)(((size_t)t * C::G * g.sx + col) * sizeof(float2));
     float2 u[C::G][C::T];
 #pragma unroll
     for (int gg = 0; gg < C::G; gg++)
 #pragma unroll
         for (int k2 = 0; k2 < C::T; k2++)
-            u[gg][k2] = ok ? src[(long)(t * C::G + gg + C::E * k2) * g.sx] : float2{0.0f, 0.0f};
+            u[gg][k2] = ld_off<float2>(src + (size_t)(gg + C::E * k2) * g.sx, roff);
     __syncthreads();
     float2 v[C::E];
     col_inverse<L, C::W, C::S>(u, v, w, t, sbuf, stw);
     if (!ok) return;
-    float2 *dst = xb + ((long)c * g.nrows + (long)i * g.n1) * g.sx + kx0 + w;
+    float2 *dst = xb + ((size_t)c * g.nrows + (size_t)i * g.n1) * g.sx;
+    const unsigned woff = (unsigned)(((size_t)t * g.sx + col) * sizeof(float2));
 #pragma unroll
-    for (int q = 0; q < C::E; q++) {
-        const int j = t + C::T * q;
-        if (j < g.n1) dst[(long)j * g.sx] = v[q];
-    }
+    for (int q = 0; q < NQ; q++) st_off<float2>(dst + (size_t)q * C::T * g.sx, woff, v[q]);
 }
 
 // z pass: three thread groups, one per field component, each transforms its
@@ -177,17 +193,6 @@ template <int L> __device__ __forceinline__ int zrow(int kz)
         return kz ^ ((kz >> ZTile<L>::SWZ) & 1);
     else
         return kz;
-}
-
-// *(T *)(base + off) with a 32-bit byte offset: with a wave-uniform base this
-// is one global access with a scalar base and a vector offset.
-template <typename T> __device__ __forceinline__ T ld_off(const void *base, unsigned off)
-{
-    return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + off);
-}
-template <typename T> __device__ __forceinline__ void st_off(void *base, unsigned off, T v)
-{
-    *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + off) = v;
 }
 
 // ky visiting order 0, p/2, 1, p-1, 2, p-2, ...: tiles of ky and p - ky
@@ -322,12 +327,14 @@ z_fast(DemagGeom g, int ntx, const float2 *__restrict__ tw, const float *__restr
 }
 
 // films (p0 == 1): y transform, kernel multiply and inverse y in one kernel
-template <int L>
+// (FULL: periodic y, n1 == L; otherwise rows n1 .. L-1 are zero padding)
+template <int L, bool FULL>
 __global__ void __launch_bounds__(ColTile<L>::NT)
 y2d_fast(DemagGeom g, int ntx, const float2 *__restrict__ tw, const float *__restrict__ K,
          float2 *__restrict__ xb)
 {
     using C = ColTile<L>;
+    constexpr int NQ = FULL ? C::E : C::E / 2;
     extern __shared__ float2 sm[];
     float2 *sbuf = sm, *stw = sm + C::E * C::S;
     const int tile = xcd_tile(blockIdx.x, gridDim.x);
@@ -335,19 +342,18 @@ y2d_fast(DemagGeom g, int ntx, const float2 *__restrict__ tw, const float *__res
     const int w = threadIdx.x % C::W, t = threadIdx.x / C::W;
     for (int q = threadIdx.x; q < L; q += C::NT) stw[q] = tw[q];
     const bool ok = kx0 + w < g.h2;
-    const long cstride = (long)g.nrows * g.sx;
-    float2 *base = xb + kx0 + w;
+    const int col = ok ? kx0 + w : g.h2 - 1;
+    const size_t cstride = (size_t)g.nrows * g.sx;
+    const unsigned off = (unsigned)(((size_t)t * g.sx + col) * sizeof(float2));
     float2 u[3][C::G][C::T];
     __syncthreads();
 #pragma unroll
     for (int c = 0; c < 3; c++) {
         float2 v[C::E];
 #pragma unroll
-        for (int q = 0; q < C::E; q++) {
-            const int j = t + C::T * q;
-            v[q] = (ok && j < g.n1) ? base[c * cstride + (long)j * g.sx] : float2{0.0f, 0.0f};
-        }
-        col_forward<L, C::W, C::S>(v, u[c], w, t, sbuf, stw);
+        for (int q = 0; q < NQ; q++)
+            v[q] = ld_off<float2>(xb + c * cstride + (size_t)q * C::T * g.sx, off);
+        col_forward<L, C::W, C::S, !FULL>(v, u[c], w, t, sbuf, stw);
     }
     if (ok) {
 #pragma unroll
@@ -364,10 +370,8 @@ y2d_fast(DemagGeom g, int ntx, const float2 *__restrict__ tw, const float *__res
         col_inverse<L, C::W, C::S>(u[c], v, w, t, sbuf, stw);
         if (ok) {
 #pragma unroll
-            for (int q = 0; q < C::E; q++) {
-                const int j = t + C::T * q;
-                if (j < g.n1) base[c * cstride + (long)j * g.sx] = v[q];
-            }
+            for (int q = 0; q < NQ; q++)
+                st_off<float2>(xb + c * cstride + (size_t)q * C::T * g.sx, off, v[q]);
         }
     }
 }
@@ -718,13 +722,15 @@ template <typename K> static void allow_lds_fast(K kernel, size_t bytes)
 int fast_yf(const DemagPlanImpl *P, hipStream_t st)
 {
     const DemagGeom &g = P->g;
+    MMX_REQUIRE(g.n1 == g.p1 || 2 * g.n1 == g.p1, MMX_ERR_SIZE, "fast_yf: rows %d of %d", g.n1, g.p1);
     switch (g.p1) {
 #define CASE(L)                                                                              \
     case L: {                                                                                \
         using C = ColTile<L>;                                                                \
         const int ntx = div_up(g.h2, C::W);                                                  \
-        allow_lds_fast(yf_fast<L>, C::LDS);                                                  \
-        yf_fast<L><<<dim3(ntx * g.n0 * 3), dim3(C::NT), C::LDS, st>>>(g, ntx, P->fy.tw,      \
+        auto kern = (g.n1 == g.p1) ? yf_fast<L, true> : yf_fast<L, false>;                   \
+        allow_lds_fast(kern, C::LDS);                                                        \
+        kern<<<dim3(ntx * g.n0 * 3), dim3(C::NT), C::LDS, st>>>(g, ntx, P->fy.tw,            \
                                                                       P->xbuf, P->ybuf);     \
         break;                                                                               \
     }
@@ -739,13 +745,15 @@ int fast_yf(const DemagPlanImpl *P, hipStream_t st)
 int fast_yi(const DemagPlanImpl *P, hipStream_t st)
 {
     const DemagGeom &g = P->g;
+    MMX_REQUIRE(g.n1 == g.p1 || 2 * g.n1 == g.p1, MMX_ERR_SIZE, "fast_yi: rows %d of %d", g.n1, g.p1);
     switch (g.p1) {
 #define CASE(L)                                                                              \
     case L: {                                                                                \
         using C = ColTile<L>;                                                                \
         const int ntx = div_up(g.h2, C::W);                                                  \
-        allow_lds_fast(yi_fast<L>, C::LDS);                                                  \
-        yi_fast<L><<<dim3(ntx * g.n0 * 3), dim3(C::NT), C::LDS, st>>>(g, ntx, P->fy.tw,      \
+        auto kern = (g.n1 == g.p1) ? yi_fast<L, true> : yi_fast<L, false>;                   \
+        allow_lds_fast(kern, C::LDS);                                                        \
+        kern<<<dim3(ntx * g.n0 * 3), dim3(C::NT), C::LDS, st>>>(g, ntx, P->fy.tw,            \
                                                                       P->ybuf, P->xbuf);     \
         break;                                                                               \
     }
@@ -803,13 +811,15 @@ int fast_z(const DemagPlanImpl *P, hipStream_t st)
 int fast_y2d(const DemagPlanImpl *P, hipStream_t st)
 {
     const DemagGeom &g = P->g;
+    MMX_REQUIRE(g.n1 == g.p1 || 2 * g.n1 == g.p1, MMX_ERR_SIZE, "fast_y2d: rows %d of %d", g.n1, g.p1);
     switch (g.p1) {
 #define CASE(L)                                                                              \
     case L: {                                                                                \
         using C = ColTile<L>;                                                                \
         const int ntx = div_up(g.h2, C::W);                                                  \
-        allow_lds_fast(y2d_fast<L>, C::LDS);                                                 \
-        y2d_fast<L><<<dim3(ntx), dim3(C::NT), C::LDS, st>>>(g, ntx, P->fy.tw, P->kern,       \
+        auto kern = (g.n1 == g.p1) ? y2d_fast<L, true> : y2d_fast<L, false>;                 \
+        allow_lds_fast(kern, C::LDS);                                                        \
+        kern<<<dim3(ntx), dim3(C::NT), C::LDS, st>>>(g, ntx, P->fy.tw, P->kern,              \
                                                             P->xbuf);                        \
         break;                                                                               \
     }

@@ -94,7 +94,10 @@ CONV_CASES = [((1, 32, 128), (3e-9, 3.90625e-9, 3.90625e-9), (0, 0, 0)),
               ((256, 4, 8), (2e-9, 4e-9, 4e-9), (0, 0, 0)),
               ((1024, 2, 4), (2e-9, 4e-9, 4e-9), (0, 0, 0)),
               ((8, 16, 16), (3e-9, 4e-9, 4e-9), (1, 0, 0)),
-              ((64, 8, 8), (3e-9, 3e-9, 3e-9), (1, 0, 0))]
+              ((64, 8, 8), (3e-9, 3e-9, 3e-9), (1, 0, 0)),
+              # periodic y at register-FFT lengths: film (y2d) and 3-D (yf, yi)
+              ((1, 64, 32), (3e-9, 4e-9, 4e-9), (0, 1, 0)),
+              ((4, 32, 16), (3e-9, 4e-9, 4e-9), (0, 1, 0))]
 
 
 @pytest.mark.parametrize("n,cell,pbc", CONV_CASES)
