@@ -381,13 +381,13 @@ y2d_fast(DemagGeom g, int ntx, const float2 *__restrict__ tw, const float *__res
 // Thread (r, t): r = tid / T (row of the tile), t = tid % T.  Transpose
 // buffer index of Y[t][k1] of row r: k1*S + r*T + t.
 
-template <int LX>
+template <int LX, bool HALFIN = false>
 __device__ __forceinline__ void row_forward(float2 (&v)[RowTile<LX>::E],
                                             float2 (&u)[RowTile<LX>::G][RowTile<LX>::T], int r,
                                             int t, float2 *sbuf, const float2 *stw)
 {
     using C = RowTile<LX>;
-    fs_first<LX, -1>(v, t, stw);
+    fs_first<LX, -1, HALFIN>(v, t, stw);
 #pragma unroll
     for (int k1 = 0; k1 < C::E; k1++) sbuf[k1 * C::S + r * C::T + t] = v[k1];
     __syncthreads();
@@ -442,8 +442,11 @@ __device__ __forceinline__ float2 mirror_conj(const float2 (&u)[RowTile<LX>::G][
     return cconj(a);
 }
 
-// xf: R2C along x of the n0*n1 data rows (half-length complex FFT)
-template <int LX, int PRED>
+// xf: R2C along x of the n0*n1 data rows (half-length complex FFT).  FULL:
+// periodic x (n2 == 2 LX, every pair holds data); otherwise n2 == LX and the
+// pairs LX/2 .. LX-1 are zero padding (not read).  Lane offsets are 32-bit
+// from the component bases (fast_x_addressable).
+template <int LX, int PRED, bool FULL>
 __global__ void __launch_bounds__(RowTile<LX>::NT)
 xf_fast(DemagGeom g, const float2 *__restrict__ tw, const float2 *__restrict__ rtw,
         const float *__restrict__ m, const float *__restrict__ msat_map,
@@ -451,64 +454,73 @@ xf_fast(DemagGeom g, const float2 *__restrict__ tw, const float2 *__restrict__ r
         float *__restrict__ mpred, float2 *__restrict__ xb)
 {
     using C = RowTile<LX>;
+    constexpr int NQ = FULL ? C::E : C::E / 2; // cell pairs 2 (t + T q), q < NQ
     extern __shared__ float2 sm[];
     float2 *sbuf = sm, *stw = sm + C::E * C::S, *srtw = stw + LX;
     const int tile = xcd_tile(blockIdx.x, gridDim.x);
     const int r = threadIdx.x / C::T, t = threadIdx.x % C::T;
-    const long row = (long)tile * C::R + r;
+    const int row = tile * C::R + r;
     const bool ok = row < g.nrows;
+    const int crow = ok ? row : (int)g.nrows - 1; // rows past the end load the last row
     for (int q = threadIdx.x; q < LX; q += C::NT) stw[q] = tw[q];
     for (int q = threadIdx.x; q <= LX; q += C::NT) srtw[q] = rtw[q];
-    const long N = g.N;
+    const size_t N = g.N;
+    const unsigned cb = (unsigned)(crow * g.n2 + 2 * t) * 4u;
     float2 v[3][C::E];
 #pragma unroll
-    for (int q = 0; q < C::E; q++) {
-        float a[2][3] = {{0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}};
-        const int n = t + C::T * q;
+    for (int q = 0; q < NQ; q++) {
+        const unsigned o = cb + (unsigned)(q * 2 * C::T * 4);
+        float a0[2], a1[2], a2[2];
+        {
+            const float2 x0 = ld_off<float2>(m, o), x1 = ld_off<float2>(m + N, o),
+                         x2 = ld_off<float2>(m + 2 * N, o);
+            a0[0] = x0.x; a0[1] = x0.y;
+            a1[0] = x1.x; a1[1] = x1.y;
+            a2[0] = x2.x; a2[1] = x2.y;
+        }
+        if (PRED) {
+            const float2 p0 = ld_off<float2>(tau0, o), p1 = ld_off<float2>(tau0 + N, o),
+                         p2 = ld_off<float2>(tau0 + 2 * N, o);
+            const float2 nm = norm_map ? ld_off<float2>(norm_map, o) : float2{1.0f, 1.0f};
+            const float pt0[2] = {p0.x, p0.y}, pt1[2] = {p1.x, p1.y}, pt2[2] = {p2.x, p2.y};
+            const float nms[2] = {nm.x, nm.y};
 #pragma unroll
-        for (int s = 0; s < 2; s++) {
-            const int k = 2 * n + s;
-            if (ok && k < g.n2) {
-                const long I = row * g.n2 + k;
-                float a0 = m[I], a1 = m[N + I], a2 = m[2 * N + I];
-                if (PRED) {
-                    a0 = a0 + h * tau0[I];
-                    a1 = a1 + h * tau0[N + I];
-                    a2 = a2 + h * tau0[2 * N + I];
-                    const float norm = norm_map ? norm_map[I] : 1.0f;
-                    const float len = sqrtf(a0 * a0 + a1 * a1 + a2 * a2);
-                    float scale = (norm == 0.0f) ? 0.0f : 1.0f;
-                    scale = (len != 0.0f) ? scale / len : 0.0f;
-                    a0 *= scale;
-                    a1 *= scale;
-                    a2 *= scale;
-                    mpred[I] = a0;
-                    mpred[N + I] = a1;
-                    mpred[2 * N + I] = a2;
-                }
-                if (msat_map) {
-                    const float ms = msat_map[I];
-                    a0 *= ms;
-                    a1 *= ms;
-                    a2 *= ms;
-                }
-                a[s][0] = a0;
-                a[s][1] = a1;
-                a[s][2] = a2;
+            for (int s = 0; s < 2; s++) {
+                a0[s] = a0[s] + h * pt0[s];
+                a1[s] = a1[s] + h * pt1[s];
+                a2[s] = a2[s] + h * pt2[s];
+                const float len = sqrtf(a0[s] * a0[s] + a1[s] * a1[s] + a2[s] * a2[s]);
+                float scale = (nms[s] == 0.0f) ? 0.0f : 1.0f;
+                scale = (len != 0.0f) ? scale / len : 0.0f;
+                a0[s] *= scale;
+                a1[s] *= scale;
+                a2[s] *= scale;
+            }
+            if (ok) {
+                st_off<float2>(mpred, o, float2{a0[0], a0[1]});
+                st_off<float2>(mpred + N, o, float2{a1[0], a1[1]});
+                st_off<float2>(mpred + 2 * N, o, float2{a2[0], a2[1]});
             }
         }
-#pragma unroll
-        for (int c = 0; c < 3; c++) v[c][q] = float2{a[0][c], a[1][c]};
+        if (msat_map) {
+            const float2 ms = ld_off<float2>(msat_map, o);
+            a0[0] *= ms.x; a1[0] *= ms.x; a2[0] *= ms.x;
+            a0[1] *= ms.y; a1[1] *= ms.y; a2[1] *= ms.y;
+        }
+        v[0][q] = float2{a0[0], a0[1]};
+        v[1][q] = float2{a1[0], a1[1]};
+        v[2][q] = float2{a2[0], a2[1]};
     }
     __syncthreads();
     float2 u[3][C::G][C::T];
 #pragma unroll
-    for (int c = 0; c < 3; c++) row_forward<LX>(v[c], u[c], r, t, sbuf, stw);
+    for (int c = 0; c < 3; c++) row_forward<LX, !FULL>(v[c], u[c], r, t, sbuf, stw);
 
     const int lane_row0 = (threadIdx.x & 63) - t;
+    const unsigned ob = (unsigned)((size_t)crow * g.sx * sizeof(float2));
 #pragma unroll
     for (int c = 0; c < 3; c++) {
-        float2 *dst = xb + ((long)c * g.nrows + row) * g.sx;
+        float2 *dst = xb + (size_t)c * g.nrows * g.sx;
         static_for<C::G * C::T>([&](auto I) {
             constexpr int gg = decltype(I)::value / C::T, k2 = decltype(I)::value % C::T;
             const float2 b = mirror_conj<LX, gg, k2>(u[c], t, lane_row0);
@@ -517,10 +529,10 @@ xf_fast(DemagGeom g, const float2 *__restrict__ tw, const float2 *__restrict__ r
             const float2 Ev = float2{0.5f * (a.x + b.x), 0.5f * (a.y + b.y)};
             const float2 Ov = float2{0.5f * (a.y - b.y), -0.5f * (a.x - b.x)};
             const float2 X = cadd(Ev, cmul(Ov, srtw[k]));
-            if (ok) dst[k] = X;
+            if (ok) st_off<float2>(dst, ob + (unsigned)k * 8u, X);
             if (ok && k == 0) {
                 // Nyquist bin: E(0) - O(0) = Re Z0 - Im Z0
-                dst[LX] = float2{a.x - a.y, 0.0f};
+                st_off<float2>(dst, ob + (unsigned)LX * 8u, float2{a.x - a.y, 0.0f});
             }
         });
     }
@@ -530,11 +542,6 @@ xf_fast(DemagGeom g, const float2 *__restrict__ tw, const float2 *__restrict__ r
 // uniform material parameters (no maps) and no spin-transfer term: the same
 // per-cell arithmetic as cell_epilogue (exchange6 order dim 0, dim 2, dim 1;
 // SumUpdater order), with 8-byte loads and stores of the cell pair.
-__device__ __forceinline__ float2 ld2(const float *p) { return *reinterpret_cast<const float2 *>(p); }
-__device__ __forceinline__ void st2(float *p, float a, float b)
-{
-    *reinterpret_cast<float2 *>(p) = float2{a, b};
-}
 
 __device__ __forceinline__ int nb_lo(int pos, int len, int wrap)
 {
@@ -551,40 +558,42 @@ __device__ __forceinline__ int nb_hi(int pos, int len, int wrap)
 
 template <int MODE>
 __device__ __forceinline__ void pair_epilogue(const DemagGeom &g, const FieldParams &fp,
-                                              const EpilogueIO &io, long I0, int i, int j, int k0,
-                                              const float hd0[3], const float hd1[3],
+                                              const EpilogueIO &io, unsigned I0, int i, int j,
+                                              int k0, const float hd0[3], const float hd1[3],
                                               float &errmax)
 {
-    const long N = g.N;
+    const size_t N = g.N;
+    const unsigned b0 = I0 * 4u; // byte offset of cell I0 in a component array
     if (MODE == EPI_B) {
 #pragma unroll
-        for (int c = 0; c < 3; c++) st2(io.hout + c * N + I0, hd0[c], hd1[c]);
+        for (int c = 0; c < 3; c++) st_off<float2>(io.hout + c * N, b0, float2{hd0[c], hd1[c]});
         return;
     }
     const float *__restrict__ m = io.m;
     float m0[3], m1[3], H0[3], H1[3];
 #pragma unroll
     for (int c = 0; c < 3; c++) {
-        const float2 a = ld2(m + c * N + I0);
+        const float2 a = ld_off<float2>(m + c * N, b0);
         m0[c] = a.x;
         m1[c] = a.y;
     }
-    // neighbour offsets: dim 0 (planes), dim 2 (along the row), dim 1 (rows)
-    const long s0 = (long)g.n1 * g.n2, s1 = g.n2;
-    const long o0l = (long)(nb_lo(i, g.n0, fp.pbc[0]) - i) * s0;
-    const long o0h = (long)(nb_hi(i, g.n0, fp.pbc[0]) - i) * s0;
-    const long o1l = (long)(nb_lo(j, g.n1, fp.pbc[1]) - j) * s1;
-    const long o1h = (long)(nb_hi(j, g.n1, fp.pbc[1]) - j) * s1;
-    const long o2l = nb_lo(k0, g.n2, fp.pbc[2]) - k0;          // left of cell 0
-    const long o2h = nb_hi(k0 + 1, g.n2, fp.pbc[2]) - (k0 + 1); // right of cell 1
+    // neighbour byte offsets: dim 0 (planes), dim 2 (along the row), dim 1
+    // (rows); negative steps wrap in the unsigned sum and land on valid cells
+    const int s0 = g.n1 * g.n2, s1 = g.n2;
+    const unsigned o0l = b0 + 4u * (unsigned)((nb_lo(i, g.n0, fp.pbc[0]) - i) * s0);
+    const unsigned o0h = b0 + 4u * (unsigned)((nb_hi(i, g.n0, fp.pbc[0]) - i) * s0);
+    const unsigned o1l = b0 + 4u * (unsigned)((nb_lo(j, g.n1, fp.pbc[1]) - j) * s1);
+    const unsigned o1h = b0 + 4u * (unsigned)((nb_hi(j, g.n1, fp.pbc[1]) - j) * s1);
+    const unsigned o2l = b0 + 4u * (unsigned)(nb_lo(k0, g.n2, fp.pbc[2]) - k0); // left of cell 0
+    const unsigned o2h = b0 + 4u * (unsigned)(nb_hi(k0 + 1, g.n2, fp.pbc[2]) - k0); // right of 1
     const float p0 = fp.ex_pre * fp.ex_c[0], p1 = fp.ex_pre * fp.ex_c[1],
                 p2 = fp.ex_pre * fp.ex_c[2];
 #pragma unroll
     for (int c = 0; c < 3; c++) {
-        const float *mc = m + c * N + I0;
-        const float2 zl = ld2(mc + o0l), zh = ld2(mc + o0h);
-        const float2 yl = ld2(mc + o1l), yh = ld2(mc + o1h);
-        const float xl = mc[o2l], xh = mc[1 + o2h];
+        const float *mc = m + c * N;
+        const float2 zl = ld_off<float2>(mc, o0l), zh = ld_off<float2>(mc, o0h);
+        const float2 yl = ld_off<float2>(mc, o1l), yh = ld_off<float2>(mc, o1h);
+        const float xl = ld_off<float>(mc, o2l), xh = ld_off<float>(mc, o2h);
         float e0 = p0 * ((zl.x - m0[c]) + (zh.x - m0[c]));
         float e1 = p0 * ((zl.y - m1[c]) + (zh.y - m1[c]));
         e0 += p2 * ((xl - m0[c]) + (m1[c] - m0[c]));
@@ -606,7 +615,7 @@ __device__ __forceinline__ void pair_epilogue(const DemagGeom &g, const FieldPar
     }
     if (io.hout) {
 #pragma unroll
-        for (int c = 0; c < 3; c++) st2(io.hout + c * N + I0, H0[c], H1[c]);
+        for (int c = 0; c < 3; c++) st_off<float2>(io.hout + c * N, b0, float2{H0[c], H1[c]});
     }
     if (MODE == EPI_HEFF && io.tout == nullptr) return;
     float t0[3], t1[3];
@@ -614,7 +623,7 @@ __device__ __forceinline__ void pair_epilogue(const DemagGeom &g, const FieldPar
     llg_torque(m1, H1, fp.alpha, t1);
     if (io.tout) {
 #pragma unroll
-        for (int c = 0; c < 3; c++) st2(io.tout + c * N + I0, t0[c], t1[c]);
+        for (int c = 0; c < 3; c++) st_off<float2>(io.tout + c * N, b0, float2{t0[c], t1[c]});
     }
     if (MODE == EPI_CORR) {
         const float hh = 0.5f * io.h, mh = -0.5f * io.h;
@@ -622,7 +631,7 @@ __device__ __forceinline__ void pair_epilogue(const DemagGeom &g, const FieldPar
         float d = 0.0f;
 #pragma unroll
         for (int c = 0; c < 3; c++) {
-            const float2 p = ld2(io.tau0 + c * N + I0);
+            const float2 p = ld_off<float2>(io.tau0 + c * N, b0);
             a0[c] = m0[c] + (hh * t0[c] + mh * p.x);
             a1[c] = m1[c] + (hh * t1[c] + mh * p.y);
             d = fmaxf(d, fmaxf(fabsf(t0[c] - p.x), fabsf(t1[c] - p.y)));
@@ -633,43 +642,47 @@ __device__ __forceinline__ void pair_epilogue(const DemagGeom &g, const FieldPar
         const float q0 = (l0 != 0.0f) ? 1.0f / l0 : 0.0f;
         const float q1 = (l1 != 0.0f) ? 1.0f / l1 : 0.0f;
 #pragma unroll
-        for (int c = 0; c < 3; c++) st2(io.mnew + c * N + I0, a0[c] * q0, a1[c] * q1);
+        for (int c = 0; c < 3; c++)
+            st_off<float2>(io.mnew + c * N, b0, float2{a0[c] * q0, a1[c] * q1});
     }
 }
 
 // xi: C2R along x (imaginary parts of DC and Nyquist ignored), keep k < n2,
 // then the fused field / torque / RK epilogue
-template <int LX, int MODE>
+template <int LX, int MODE, bool FULL>
 __global__ void __launch_bounds__(RowTile<LX>::NT)
 xi_fast(DemagGeom g, const float2 *__restrict__ tw, const float2 *__restrict__ rtw,
         const float2 *__restrict__ xb, FieldParams fp, EpilogueIO io)
 {
     using C = RowTile<LX>;
+    constexpr int NQ = FULL ? C::E : C::E / 2; // cell pairs 2 (t + T q) < n2
     extern __shared__ float2 sm[];
     float2 *sbuf = sm, *stw = sm + C::E * C::S, *srtw = stw + LX;
     const int tile = xcd_tile(blockIdx.x, gridDim.x);
     const int r = threadIdx.x / C::T, t = threadIdx.x % C::T;
-    const long row = (long)tile * C::R + r;
+    const int row = tile * C::R + r;
     const bool ok = row < g.nrows;
+    const int crow = ok ? row : (int)g.nrows - 1;
     for (int q = threadIdx.x; q < LX; q += C::NT) stw[q] = tw[q];
     for (int q = threadIdx.x; q <= LX; q += C::NT) srtw[q] = rtw[q];
     __syncthreads();
     const int lane_row0 = (threadIdx.x & 63) - t;
+    const unsigned ob = (unsigned)((size_t)crow * g.sx * sizeof(float2));
     float2 v[3][C::E];
 #pragma unroll
     for (int c = 0; c < 3; c++) {
-        const float2 *src = xb + ((long)c * g.nrows + (ok ? row : 0)) * g.sx;
+        const float2 *src = xb + (size_t)c * g.nrows * g.sx;
         float2 u[C::G][C::T];
 #pragma unroll
         for (int gg = 0; gg < C::G; gg++)
 #pragma unroll
             for (int k2 = 0; k2 < C::T; k2++) {
                 const int k = t * C::G + gg + C::E * k2;
-                float2 x = ok ? src[k] : float2{0.0f, 0.0f};
+                float2 x = ld_off<float2>(src, ob + (unsigned)k * 8u);
                 if (k == 0) x.y = 0.0f;
                 u[gg][k2] = x;
             }
-        const float2 nyq = ok ? float2{src[LX].x, 0.0f} : float2{0.0f, 0.0f};
+        const float2 nyq = float2{ld_off<float2>(src, ob + (unsigned)LX * 8u).x, 0.0f};
         float2 z[C::G][C::T];
         static_for<C::G * C::T>([&](auto I) {
             constexpr int gg = decltype(I)::value / C::T, k2 = decltype(I)::value % C::T;
@@ -687,22 +700,22 @@ xi_fast(DemagGeom g, const float2 *__restrict__ tw, const float2 *__restrict__ r
     float errmax = 0.0f;
     // this kernel serves uniform parameters without the spin-transfer term
     // (fast_xi_applies); cell pairs with 8-byte I/O (n2 is even on this path:
-    // p2 = 2 LX is padded from n2 = LX, or periodic with n2 = 2 LX)
+    // n2 = LX padded to 2 LX, or periodic with n2 = 2 LX)
     if (ok) {
-        const int i = (int)(row / g.n1), j = (int)(row - (long)i * g.n1);
+        const int i = row / g.n1, j = row - i * g.n1;
 #pragma unroll
-        for (int q = 0; q < C::E; q++) {
+        for (int q = 0; q < NQ; q++) {
             // keep each cell pair's memory traffic inside its iteration
             __builtin_amdgcn_sched_barrier(0);
             const int k0 = 2 * (t + C::T * q);
-            if (k0 >= g.n2) continue;
             float hd0[3], hd1[3];
 #pragma unroll
             for (int c = 0; c < 3; c++) {
                 hd0[c] = fp.demag_scale * v[c][q].x;
                 hd1[c] = fp.demag_scale * v[c][q].y;
             }
-            pair_epilogue<MODE>(g, fp, io, row * g.n2 + k0, i, j, k0, hd0, hd1, errmax);
+            pair_epilogue<MODE>(g, fp, io, (unsigned)(row * g.n2 + k0), i, j, k0, hd0, hd1,
+                                errmax);
         }
     }
     if (MODE == EPI_CORR) block_max_to(errmax, io.err);
@@ -831,22 +844,36 @@ int fast_y2d(const DemagPlanImpl *P, hipStream_t st)
     return MMX_OK;
 }
 
+// The x passes address cells with 32-bit byte offsets from the component
+// bases, and spectrum rows likewise; n2 is LX (zero-padded) or 2 LX (periodic).
+bool fast_x_addressable(const DemagGeom &g)
+{
+    const double lim = 4294967295.0;
+    return (double)g.N * sizeof(float) < lim &&
+           ((double)g.nrows * g.sx + g.sx) * sizeof(float2) < lim &&
+           (g.n2 == g.Lx || g.n2 == 2 * g.Lx);
+}
+
 int fast_xf(const DemagPlanImpl *P, int pred, const float *m, const float *msat_map,
             const float *tau0, float h, const float *norm_map, float *mpred, hipStream_t st)
 {
     const DemagGeom &g = P->g;
+    MMX_REQUIRE(fast_x_addressable(g), MMX_ERR_SIZE, "fast_xf: unsupported geometry");
     switch (g.Lx) {
 #define CASE(L)                                                                              \
     case L: {                                                                                \
         using C = RowTile<L>;                                                                \
         const int nb = div_up(g.nrows, C::R);                                                \
+        const bool full = g.n2 == 2 * L;                                                     \
         if (pred) {                                                                          \
-            allow_lds_fast(xf_fast<L, 1>, C::LDS);                                           \
-            xf_fast<L, 1><<<dim3(nb), dim3(C::NT), C::LDS, st>>>(                            \
+            auto kern = full ? xf_fast<L, 1, true> : xf_fast<L, 1, false>;                   \
+            allow_lds_fast(kern, C::LDS);                                                    \
+            kern<<<dim3(nb), dim3(C::NT), C::LDS, st>>>(                                     \
                 g, P->fx.tw, P->fx.rtw, m, msat_map, tau0, h, norm_map, mpred, P->xbuf);     \
         } else {                                                                             \
-            allow_lds_fast(xf_fast<L, 0>, C::LDS);                                           \
-            xf_fast<L, 0><<<dim3(nb), dim3(C::NT), C::LDS, st>>>(                            \
+            auto kern = full ? xf_fast<L, 0, true> : xf_fast<L, 0, false>;                   \
+            allow_lds_fast(kern, C::LDS);                                                    \
+            kern<<<dim3(nb), dim3(C::NT), C::LDS, st>>>(                                     \
                 g, P->fx.tw, P->fx.rtw, m, msat_map, nullptr, 0.0f, nullptr, nullptr,        \
                 P->xbuf);                                                                    \
         }                                                                                    \
@@ -866,9 +893,9 @@ static void launch_xi(const DemagPlanImpl *P, const FieldParams &fp, const Epilo
 {
     using C = RowTile<L>;
     const int nb = div_up(P->g.nrows, C::R);
-    allow_lds_fast(xi_fast<L, MODE>, C::LDS);
-    xi_fast<L, MODE><<<dim3(nb), dim3(C::NT), C::LDS, st>>>(P->g, P->fx.tw, P->fx.rtw, P->xbuf,
-                                                            fp, io);
+    auto kern = (P->g.n2 == 2 * L) ? xi_fast<L, MODE, true> : xi_fast<L, MODE, false>;
+    allow_lds_fast(kern, C::LDS);
+    kern<<<dim3(nb), dim3(C::NT), C::LDS, st>>>(P->g, P->fx.tw, P->fx.rtw, P->xbuf, fp, io);
 }
 
 bool fast_xi_applies(const FieldParams &fp)
@@ -880,6 +907,7 @@ int fast_xi(const DemagPlanImpl *P, int mode, const FieldParams &fp, const Epilo
             hipStream_t st)
 {
     const DemagGeom &g = P->g;
+    MMX_REQUIRE(fast_x_addressable(g), MMX_ERR_SIZE, "fast_xi: unsupported geometry");
     switch (g.Lx) {
 #define CASE(L)                                                                              \
     case L:                                                                                  \

@@ -109,6 +109,7 @@ int fast_xf(const DemagPlanImpl *P, int pred, const float *m, const float *msat_
 int fast_yf(const DemagPlanImpl *P, hipStream_t st);
 int fast_z(const DemagPlanImpl *P, hipStream_t st);
 bool fast_z_addressable(const DemagGeom &g);
+bool fast_x_addressable(const DemagGeom &g);
 int fast_yi(const DemagPlanImpl *P, hipStream_t st);
 int fast_y2d(const DemagPlanImpl *P, hipStream_t st);
 int fast_xi(const DemagPlanImpl *P, int mode, const FieldParams &fp, const EpilogueIO &io,
