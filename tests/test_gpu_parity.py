@@ -97,7 +97,10 @@ CONV_CASES = [((1, 32, 128), (3e-9, 3.90625e-9, 3.90625e-9), (0, 0, 0)),
               ((64, 8, 8), (3e-9, 3e-9, 3e-9), (1, 0, 0)),
               # periodic y at register-FFT lengths: film (y2d) and 3-D (yf, yi)
               ((1, 64, 32), (3e-9, 4e-9, 4e-9), (0, 1, 0)),
-              ((4, 32, 16), (3e-9, 4e-9, 4e-9), (0, 1, 0))]
+              ((4, 32, 16), (3e-9, 4e-9, 4e-9), (0, 1, 0)),
+              # periodic y of length 6 under a register-FFT z pass (tile pairs
+              # ky, p1 - ky around p1/2 = 3)
+              ((4, 6, 8), (3e-9, 4e-9, 4e-9), (0, 1, 0))]
 
 
 @pytest.mark.parametrize("n,cell,pbc", CONV_CASES)
