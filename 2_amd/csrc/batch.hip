@@ -368,6 +368,8 @@ int load_sims(BatchImpl &B, double duration, long long nsteps)
     const int ns = (int)B.sims.size();
     for (int s = 0; s < ns; s++) {
         mmx_sim *x = B.sims[s];
+        MMX_REQUIRE(x->solver == 0, MMX_ERR_ARG,
+                    "batch: simulation %d has another solver than solver/rk12 loaded", s);
         MMX_TRY(sim_prepare(x));
         MMX_HIP(hipStreamSynchronize(x->st));
         const DemagPlanImpl &P = x->plan->impl;

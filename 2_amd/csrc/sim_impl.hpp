@@ -27,14 +27,20 @@ struct mmx_sim {
     long step = 0, badsteps = 0, evals = 0;
     bool m_normalized = false;
 
-    int solver = 0; // 0 solver/rk12, 1 solver/heun, 2 solver/euler
+    // 0 solver/rk12, 1 solver/heun, 2 solver/euler, 3 solver/bdf-euler, 4 solver/am12
+    int solver = 0;
+    // implicit solvers (solve_euler_backward.go:104-146, solve_am01.go:328-398)
+    int bdf_maxiter = 3;
+    double bdf_maxitererr = 1e-5, am_abserr = 1e-4, am_relerr = 1e-3;
+    long bdf_iters = 0;   // bdf_iterations of the last step
+    double am_alpha = 0;  // the predictor's last convergence ratio
     int kernel_type = MMX_KERNEL_ARNE, accuracy = 6;
     mmx_demag_plan *plan = nullptr;
     mmx_reductor *red = nullptr;
     float *m = nullptr, *mnew = nullptr, *mpred = nullptr, *tau0 = nullptr, *heff = nullptr,
           *torq = nullptr, *bf = nullptr;
-    unsigned int *derr = nullptr;
-    unsigned int *herr = nullptr; // pinned
+    unsigned int *derr = nullptr; // [0]: RK error; [4 .. 12]: implicit-solver maxima
+    unsigned int *herr = nullptr; // pinned, same layout
     hipStream_t st = nullptr;
 
     // learned inner-stage field (borrowed; mmx_sim_set_surrogate)
@@ -62,4 +68,10 @@ int surrogate_rel_l2(mmx_surrogate *S, const float *a, const float *b, const flo
 float *surrogate_hd_buffer(mmx_surrogate *S);
 // The epilogue parameters of the simulation's current settings (H_eff mode).
 FieldParams sim_field_params(const mmx_sim *s);
+// One fixed-point pass of the implicit solvers (implicit.hip): the next
+// iterate and, per component c, res[3c] = max|m - b|, res[3c+1] = max|b|,
+// res[3c+2] = max|m + b| (synchronises).
+int implicit_pass(int mode, const float *y0, const float *dy, const float *dy0, float h,
+                  const float *m, float *yb, long N, unsigned int *dout, unsigned int *hout,
+                  hipStream_t st, double res[9]);
 } // namespace mmx

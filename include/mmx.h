@@ -212,16 +212,23 @@ int mmx_sim_create(mmx_sim **sim, int n0, int n1, int n2, double cell0,
 int mmx_sim_destroy(mmx_sim *sim);
 /* Engine.LoadModule: "anisotropy/uniaxial", "zhang-li", and the solvers
  * "solver/rk12" (default, adaptive), "solver/heun" and "solver/euler" (fixed
- * dt; solve_heun.go:27-79, solve_euler.go:22-50), which replace each other
- * ("micromagnetism", "demag", "exchange6", "zeeman", "llg", "maxtorque" are
- * already loaded). */
+ * dt; solve_heun.go:27-79, solve_euler.go:22-50), "solver/bdf-euler" (fixed
+ * dt, fixed-point backward Euler; solve_euler_backward.go:29-91) and
+ * "solver/am12" (adaptive implicit-Euler predictor / trapezoidal corrector;
+ * solve_am01.go:45-288; loading it sets mindt = 1e-38, maxdt = 1e38 as the
+ * module does), which replace each other ("micromagnetism", "demag",
+ * "exchange6", "zeeman", "llg", "maxtorque" are already loaded).  Names are
+ * case-insensitive, as the reference's quantity names are. */
 int mmx_sim_load(mmx_sim *sim, const char *module);
 /* Demag kernel choice (default MMX_KERNEL_ARNE, accuracy 6); a change
  * rebuilds the plan at the next update. */
 int mmx_sim_set_kernel(mmx_sim *sim, int kernel_type, int accuracy);
 /* setv/getv on VALUE/MASK multipliers: Msat, Aex, alpha, gamma, B_ext[3],
- * Ku, anisU[3], j[3], polarisation, xi, t, dt, mindt, maxdt, m_maxerror;
- * read-only: step, badsteps, m_error, m_peakerror, <m>[3], maxtorque,
+ * Ku, anisU[3], j[3], polarisation, xi, t, dt, mindt, maxdt, m_maxerror,
+ * m_maxIterations and m_maxIterError (bdf-euler; default 3, 1e-5),
+ * m_maxAbsError and m_maxRelError (am12; default 1e-4, 1e-3);
+ * read-only: step, badsteps (am12: of the last step), bdf_iterations (of the
+ * last step), m_error, m_peakerror, <m>[3], maxtorque,
  * evals (H_eff evaluations so far), and the energies of micromag/energy
  * (modules/mod_energy.go:24-100): E_zeeman, E_ex, E_demag, E_anis and their
  * sum E, in joule, as FP64 sums with compensated partials. */
@@ -243,7 +250,10 @@ int mmx_sim_stream(mmx_sim *sim, void **stream);
 /* Quant.Update of H_eff (one effective-field evaluation), torque or B.
  * Asynchronous: queues the kernels and returns. */
 int mmx_sim_update(mmx_sim *sim, const char *name);
-/* Engine.Step / Steps / Run (solver/rk12). */
+/* Engine.Step / Steps / Run with the loaded solver.  A solver that cannot
+ * converge returns MMX_ERR_CONVERGE where the reference panics (am12 also
+ * after 1000 step cuts within one step, where the reference retries
+ * without bound). */
 int mmx_sim_step(mmx_sim *sim);
 int mmx_sim_steps(mmx_sim *sim, int nsteps);
 int mmx_sim_run(mmx_sim *sim, double duration);

@@ -358,6 +358,48 @@ def test_step_counts_and_errors(torch_gpu):
         mmx.DemagPlan((1, 8, 11), (1e-9,) * 3)   # padded 22 -> FFT 11: prime > 7
 
 
+
+@pytest.mark.parametrize("solver,dt,nstep", [("solver/bdf-euler", 1e-14, 8),
+                                             ("solver/am12", 1e-13, 6)])
+def test_implicit_solvers_against_oracle(torch_gpu, solver, dt, nstep):
+    """solver/bdf-euler (solve_euler_backward.go:29-91) and solver/am12
+    (solve_am01.go:45-288) replayed by the oracle's restatement
+    (oracle/solvers.py) on the oracle's H_eff and torque: m after every step
+    (1e-5), t and dt, and the exact iteration and bad-step counts."""
+    from oracle import solvers as OS
+    n, cell = (2, 16, 32), (3e-9, 4e-9, 4e-9)
+    _say("implicit", solver, n, dt)
+    sim, prob, _ = _problem_pair(n, cell, zl=True, bext=(0.0, 0.004, 0.009))
+    # a smooth state: from a random one the fixed-point iterations of
+    # BDF-Euler need more than m_maxIterations passes at any useful dt
+    i, j, k = np.meshgrid(*(np.arange(x) for x in n), indexing="ij")
+    m = O.normalize(np.stack([0.3 * np.cos(2 * np.pi * k / n[2]) + 0.1 * i,
+                              0.4 * np.sin(2 * np.pi * j / n[1]), np.ones(n)])).astype(np.float32)
+    sim.setarray("m", m)
+    sim.load(solver)
+    sim.setv("dt", dt)
+    rhs = OS.llg_rhs(prob)
+    o = OS.BDFEuler(dt) if solver == "solver/bdf-euler" else OS.AM12(dt)
+    mo = m.astype(np.float64)
+    for k in range(nstep):
+        sim.step()
+        mo = o.step(mo, rhs, O.GAMMA0)
+        got_t, got_dt = sim.getv("t"), sim.getv("dt")
+        _say("step %d t %.6e/%.6e dt %.6e/%.6e iters %d/%d" % (
+            k, got_t, o.t, got_dt, o.dt, sim.getv("bdf_iterations"), o.iterations))
+        assert sim.getv("bdf_iterations") == o.iterations
+        # AM12's step sizes come from ratios of FP32 maxima of |dm| ~ 1e-5
+        # (relative precision ~ 1e-3 with m ~ 1 in float32), as in the reference
+        rel = 1e-9 if solver == "solver/bdf-euler" else 2e-3
+        assert abs(got_t - o.t) <= rel * o.t
+        assert abs(got_dt - o.dt) <= rel * o.dt
+        if solver == "solver/am12":
+            assert sim.getv("badsteps") == o.badsteps
+        assert sim.getv("step") == k + 1
+        sim.getv("<m>")              # normalises m on both sides, as the next update does
+        got = sim.getarray("m")
+        assert np.abs(got - O.normalize(mo)).max() < 1e-5
+
 @pytest.mark.parametrize("solver", ["solver/heun", "solver/euler"])
 def test_fixed_step_solvers(torch_gpu, solver):
     """solver/heun (solve_heun.go:27-79) = one predictor/corrector try at the

@@ -75,6 +75,50 @@ def test_sp4_against_standard_solution(api):
     assert late < 0.35
 
 
+def test_sp4_am12_against_standard_solution(api):
+    """examples/stdproblem4-am01.py: SP4 with solver/am12 (absolute error 1e-4,
+    relative 1e-3, dt in [1e-17, 1e-10] s), relaxed at alpha = 1 (2 ns, as the
+    RK12 test), then the reversal at alpha = 0.02 for 1 ns against
+    tests/stdsolution4.txt with the same bounds as the RK12 run."""
+    _say("SP4 am12")
+    ref = np.loadtxt(os.path.join(GOLDEN, "stdsolution4.txt"))
+    e = api.reset()
+    e.setgridsize(128, 32, 1)
+    e.setcellsize(500e-9 / 128, 125e-9 / 32, 3e-9)
+    for mod in ("exchange6", "demag", "zeeman", "llg", "maxtorque", "solver/am12"):
+        e.load(mod)
+    e.setv("m_maxabserror", 1e-4)
+    e.setv("m_maxrelerror", 1e-3)
+    e.setv("maxdt", 1e-10)
+    e.setv("mindt", 1e-17)
+    e.setv("Msat", 800e3)
+    e.setv("Aex", 1.3e-11)
+    e.setv("alpha", 1)
+    e.setv("dt", 1e-12)
+    e.setarray("m", [[[[1]]], [[[1]]], [[[0]]]])
+    e.run(2e-9)
+    m0 = e.getv("<m>")
+    _say("SP4 am12 relaxed <m>", m0, "steps", e.getv("step"))
+    assert abs(m0[0] - ref[0, 1]) < 0.01 and abs(m0[1] - ref[0, 2]) < 0.01
+    e.setv("B_ext", [-24.6e-3, 4.3e-3, 0.0])
+    e.setv("alpha", 0.02)
+    e.setv("dt", 1e-15)
+    e.setv("t", 0.0)
+    tab = e.autotabulate(["t", "<m>"], period=10e-12)
+    e.run(1e-9)
+    rows = np.array(tab.rows)
+    dev = []
+    for t, mx, my, mz in rows:
+        want = [np.interp(t * 1e9, ref[:, 0], ref[:, c]) for c in (1, 2, 3)]
+        dev.append((t, max(abs(mx - want[0]), abs(my - want[1]), abs(mz - want[2]))))
+    dev = np.array(dev)
+    early = dev[dev[:, 0] <= 0.12e-9, 1].max()
+    late = dev[:, 1].max()
+    _say("SP4 am12 max |d<m>| t<=0.12ns %.4f, all %.4f, steps %d" % (early, late, e.getv("step")))
+    assert early < 0.03
+    assert late < 0.35
+
+
 def test_sp5_against_reference_trace(api):
     """examples/stdproblem5.py: vortex relaxed at alpha = 1 (3 ns: the
     reference trace starts from a fully centred vortex, which the script's
