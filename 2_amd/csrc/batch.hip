@@ -370,6 +370,8 @@ int load_sims(BatchImpl &B, double duration, long long nsteps)
         mmx_sim *x = B.sims[s];
         MMX_REQUIRE(x->solver == 0, MMX_ERR_ARG,
                     "batch: simulation %d has another solver than solver/rk12 loaded", s);
+        MMX_REQUIRE(!x->thermal, MMX_ERR_ARG,
+                    "batch: simulation %d has temperature/brown loaded", s);
         MMX_TRY(sim_prepare(x));
         MMX_HIP(hipStreamSynchronize(x->st));
         const DemagPlanImpl &P = x->plan->impl;

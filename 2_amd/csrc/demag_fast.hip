@@ -900,7 +900,7 @@ static void launch_xi(const DemagPlanImpl *P, const FieldParams &fp, const Epilo
 
 bool fast_xi_applies(const FieldParams &fp)
 {
-    return !fp.msat_map && !fp.aex_map && !fp.alpha_map && !fp.ku_map && !fp.zl;
+    return !fp.msat_map && !fp.aex_map && !fp.alpha_map && !fp.ku_map && !fp.zl && !fp.htherm;
 }
 
 int fast_xi(const DemagPlanImpl *P, int mode, const FieldParams &fp, const EpilogueIO &io,

@@ -47,6 +47,7 @@ struct FieldParams {
     const float *zl_jmask;      // 3 components or NULL
     float zl_step[3];           // 1/(12 cell)
     float inv_gamma;            // weight of zzt in the torque sum
+    const float *htherm = nullptr; // temperature/brown field (3 N) or NULL
 };
 
 // Per-launch state of the epilogue (pointers that change between calls).

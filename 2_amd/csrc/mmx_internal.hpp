@@ -57,6 +57,7 @@ constexpr double kMu0 = 4.0 * kPi * 1e-7;
 constexpr double kGamma0 = 2.211e5;
 constexpr double kMuB = 9.2740091523e-24;
 constexpr double kQe = 1.60217646e-19;
+constexpr double kKb = 1.380650424e-23; // common/constants.go:21
 
 inline int div_up(long a, long b) { return (int)((a + b - 1) / b); }
 

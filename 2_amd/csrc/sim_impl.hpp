@@ -34,6 +34,18 @@ struct mmx_sim {
     double bdf_maxitererr = 1e-5, am_abserr = 1e-4, am_relerr = 1e-3;
     long bdf_iters = 0;   // bdf_iterations of the last step
     double am_alpha = 0;  // the predictor's last convergence ratio
+
+    // temperature/brown (mod_temperature.go:22-134): Temp multiplier and map,
+    // Therm_seed, cutoff_dt; the field and the state of its updater
+    bool thermal = false;
+    double temp = 0, therm_seed = 0, cutoff_dt = 0;
+    float *temp_map = nullptr, *htherm = nullptr;
+    unsigned long long therm_draws = 0; // draws since the seed was set
+    long long therm_seed_cache = -10000000000LL;
+    double therm_last_time = 0;
+    bool therm_valid = false;           // drawn for (therm_step, therm_dt, therm_key)
+    long therm_step = -1;
+    double therm_dt = 0, therm_key[4] = {0, 0, 0, 0};
     int kernel_type = MMX_KERNEL_ARNE, accuracy = 6;
     mmx_demag_plan *plan = nullptr;
     mmx_reductor *red = nullptr;
@@ -68,6 +80,11 @@ int surrogate_rel_l2(mmx_surrogate *S, const float *a, const float *b, const flo
 float *surrogate_hd_buffer(mmx_surrogate *S);
 // The epilogue parameters of the simulation's current settings (H_eff mode).
 FieldParams sim_field_params(const mmx_sim *s);
+// Fills out (3 N) with the temperature/brown field of draw `draw` of `seed`
+// (thermal.hip); a = float32(alpha 2 kB T), b = float32(mu0 V gamma dt Msat).
+int therm_draw(unsigned long long seed, unsigned long long draw, long N, const float *alpha_map,
+               const float *temp_map, const float *msat_map, float a, float b, float *out,
+               hipStream_t st);
 // One fixed-point pass of the implicit solvers (implicit.hip): the next
 // iterate and, per component c, res[3c] = max|m - b|, res[3c+1] = max|b|,
 // res[3c+2] = max|m + b| (synchronises).

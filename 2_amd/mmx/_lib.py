@@ -63,6 +63,7 @@ _SIGS = {
     "mmx_sim_try_step": (_i, [_vp, _d, C.POINTER(_d)]),
     "mmx_sim_time_passes": (_i, [_vp, _i, C.POINTER(_d), C.POINTER(_i)]),
     "mmx_crc64_iso": (C.c_ulonglong, [_vp, C.c_ulonglong, C.c_ulonglong]),
+    "mmx_philox4x32_10": (C.c_int, [_vp, _vp, _vp]),
     "mmx_surrogate_create": (_i, [C.POINTER(_vp), _vp, C.c_ulonglong]),
     "mmx_surrogate_destroy": (_i, [_vp]),
     "mmx_surrogate_num_params": (_i, [_vp, C.POINTER(_i)]),

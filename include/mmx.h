@@ -214,6 +214,8 @@ int mmx_sim_destroy(mmx_sim *sim);
  * "solver/rk12" (default, adaptive), "solver/heun" and "solver/euler" (fixed
  * dt; solve_heun.go:27-79, solve_euler.go:22-50), "solver/bdf-euler" (fixed
  * dt, fixed-point backward Euler; solve_euler_backward.go:29-91) and
+ * "temperature/brown" (Brown's fluctuating field H_therm added to H_eff,
+ * mod_temperature.go:22-134; see mmx_philox4x32_10 for the noise),
  * "solver/am12" (adaptive implicit-Euler predictor / trapezoidal corrector;
  * solve_am01.go:45-288; loading it sets mindt = 1e-38, maxdt = 1e38 as the
  * module does), which replace each other ("micromagnetism", "demag",
@@ -226,20 +228,21 @@ int mmx_sim_set_kernel(mmx_sim *sim, int kernel_type, int accuracy);
 /* setv/getv on VALUE/MASK multipliers: Msat, Aex, alpha, gamma, B_ext[3],
  * Ku, anisU[3], j[3], polarisation, xi, t, dt, mindt, maxdt, m_maxerror,
  * m_maxIterations and m_maxIterError (bdf-euler; default 3, 1e-5),
- * m_maxAbsError and m_maxRelError (am12; default 1e-4, 1e-3);
+ * m_maxAbsError and m_maxRelError (am12; default 1e-4, 1e-3), Temp,
+ * Therm_seed and cutoff_dt (temperature/brown);
  * read-only: step, badsteps (am12: of the last step), bdf_iterations (of the
- * last step), m_error, m_peakerror, <m>[3], maxtorque,
+ * last step), m_error, m_peakerror, <m>[3], <H_therm>[3], maxtorque,
  * evals (H_eff evaluations so far), and the energies of micromag/energy
  * (modules/mod_energy.go:24-100): E_zeeman, E_ex, E_demag, E_anis and their
  * sum E, in joule, as FP64 sums with compensated partials. */
 int mmx_sim_setv(mmx_sim *sim, const char *name, const double *v, int ncomp);
 int mmx_sim_getv(mmx_sim *sim, const char *name, double *v, int ncomp);
 /* setmask: spatial maps (host float32, N per component) for Msat, Aex,
- * alpha, Ku, j (3 comps); NULL host pointer removes the map. */
+ * alpha, Ku, j (3 comps), Temp; NULL host pointer removes the map. */
 int mmx_sim_setmask(mmx_sim *sim, const char *name, const float *host,
                     int ncomp);
 /* setarray/getarray (host float32, SoA, internal order): m (set/get),
- * H_eff, torque, B (get; updated first, like Quant.Update). */
+ * H_eff, torque, B, H_therm (get; updated first, like Quant.Update). */
 int mmx_sim_setarray(mmx_sim *sim, const char *name, const float *host,
                      int ncomp);
 int mmx_sim_getarray(mmx_sim *sim, const char *name, float *host, int ncomp);
@@ -345,6 +348,18 @@ int mmx_batch_tune(mmx_batch *batch, int reps, int *lds_kb);
 int mmx_batch_set_lds(mmx_batch *batch, const int *lds_kb);
 /* Workgroups per pass launch and dynamic LDS bytes per workgroup (5 each). */
 int mmx_batch_info(const mmx_batch *batch, int *nblocks, int *lds_bytes);
+
+/* ------------------------------------------------------------------------
+ * Counter-based random numbers of temperature/brown.  No counterpart in the
+ * reference, which draws its noise from cuRAND's default generator
+ * (mod_temperature.go:60-106); here H_therm is a function of (Therm_seed,
+ * draw, cell) through Philox4x32-10 (Salmon et al., SC 2011), and this is
+ * that block function on the host: 4 counter words and 2 key words in, 4
+ * words out.  H_therm of draw d (the d-th field drawn since Therm_seed was
+ * set) in cell i uses counter (i lo, i hi, d lo, d hi) and key (seed lo,
+ * seed hi); words 0/1 and 2/3 give Box-Muller pairs (FP64) for components
+ * 0, 1 and 2. */
+int mmx_philox4x32_10(const unsigned int ctr[4], const unsigned int key[2], unsigned int out[4]);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,149 @@
+"""temperature/brown on the GPU, through the C ABI, against oracle/thermal.py.
+
+* the library's Philox4x32-10 block function against the Random123
+  known-answer vectors;
+* H_therm of a draw (Temp, alpha and Msat maps, a zero-Msat cell) equals the
+  oracle's field of the same (seed, draw), to float32 rounding of the FP64
+  Box-Muller step;
+* solver/heun with the thermal field replayed by the oracle (draw k in step
+  k, the field held over the step): m after every step to 1e-5;
+* a new Therm_seed restarts the draws; Temp = 0 removes the field.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+from oracle import oracle as O  # noqa: E402
+from oracle import thermal as T  # noqa: E402
+
+CELL = (3e-9, 4e-9, 4e-9)
+N3 = (2, 16, 32)
+
+
+def _say(*a):
+    print("[case]", *a, flush=True)
+
+
+@pytest.fixture(scope="module")
+def mmx_gpu():
+    import torch
+    assert torch.cuda.is_available(), "GPU tests need a GPU"
+    import mmx
+    mmx.check(mmx.lib().mmx_init(0))
+    return mmx
+
+
+# Random123 kat_vectors for philox4x32 with 10 rounds
+KAT = [((0, 0, 0, 0), (0, 0), (0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8)),
+       ((0xFFFFFFFF,) * 4, (0xFFFFFFFF,) * 2, (0x408F276D, 0x41C83B0E, 0xA20BC7C6, 0x6D5451FD)),
+       ((0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344), (0xA4093822, 0x299F31D0),
+        (0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1))]
+
+
+def test_library_philox_known_answers(mmx_gpu):
+    f = mmx_gpu.lib().mmx_philox4x32_10
+    for ctr, key, want in KAT:
+        c = (C.c_uint * 4)(*ctr)
+        k = (C.c_uint * 2)(*key)
+        out = (C.c_uint * 4)()
+        assert f(c, k, out) == 0
+        assert tuple(out) == want
+
+
+def _maps():
+    rng = np.random.default_rng(2)
+    tm = rng.uniform(0.5, 1.5, N3).astype(np.float32)
+    am = rng.uniform(0.5, 2.0, N3).astype(np.float32)
+    mm = np.ones(N3, np.float32)
+    mm[0, 3, 5] = 0.0
+    mm[1, 7, :4] = 0.5
+    return tm, am, mm
+
+
+def _sim(mmx, maps=True, temp=300.0, seed=7, alpha=0.1):
+    s = mmx.Sim(N3, CELL)
+    s.setv("Msat", 8e5)
+    s.setv("Aex", 1.3e-11)
+    s.setv("alpha", alpha)
+    s.load("temperature/brown")
+    s.setv("Temp", temp)
+    s.setv("Therm_seed", seed)
+    if maps:
+        tm, am, mm = _maps()
+        s.setmask("Temp", tm)
+        s.setmask("alpha", am)
+        s.setmask("Msat", mm)
+    return s
+
+
+def test_field_matches_oracle(mmx_gpu):
+    _say("H_therm draw 0", N3)
+    s = _sim(mmx_gpu)
+    s.setv("dt", 2e-14)
+    rng = np.random.default_rng(5)
+    s.setarray("m", O.normalize(rng.normal(size=(3,) + N3)).astype(np.float32))
+    got = s.getarray("H_therm")
+    tm, am, mm = _maps()
+    want = T.htherm(N3, CELL, 7, 0, 300.0, 0.1, 8e5, O.GAMMA0, 2e-14, temp_map=tm,
+                    alpha_map=am, msat_map=mm)
+    scale = np.abs(want).max()
+    _say("max |dH| / max|H| = %.2e, std %.4e" % (np.abs(got - want).max() / scale, want.std()))
+    assert np.abs(got - want).max() <= 1e-6 * scale
+    assert np.all(got[:, 0, 3, 5] == 0)
+    # no redraw within the step
+    assert np.array_equal(s.getarray("H_therm"), got)
+    # a new seed restarts the draws
+    s.setv("Therm_seed", 9)
+    want9 = T.htherm(N3, CELL, 9, 0, 300.0, 0.1, 8e5, O.GAMMA0, 2e-14, temp_map=tm,
+                     alpha_map=am, msat_map=mm)
+    assert np.abs(s.getarray("H_therm") - want9).max() <= 1e-6 * scale
+    # Temp = 0: no field
+    s.setv("Temp", 0.0)
+    assert not np.any(s.getarray("H_therm"))
+    s.close()
+
+
+def test_heun_with_thermal_field_against_oracle(mmx_gpu):
+    _say("heun + H_therm", N3)
+    dt = 1e-14
+    s = _sim(mmx_gpu, maps=False, seed=3)
+    s.load("solver/heun")
+    s.setv("dt", dt)
+    s.setv("B_ext", (0.0, 0.004, 0.009))
+    rng = np.random.default_rng(6)
+    m = O.normalize(rng.normal(size=(3,) + N3))
+    s.setarray("m", m.astype(np.float32))
+    prob = O.Problem(N3, CELL, msat=8e5, aex=1.3e-11, alpha=0.1, bext=(0.0, 0.004, 0.009))
+    mo = m.astype(np.float32).astype(np.float64)
+    for k in range(6):
+        s.step()
+        ht = T.htherm(N3, CELL, 3, k, 300.0, 0.1, 8e5, O.GAMMA0, dt).astype(np.float64)
+        mo = T.heun_step(prob, mo, dt, ht, O.GAMMA0)
+        got = s.getarray("m")
+        err = np.abs(got - mo).max()
+        _say("step %d max |dm| %.2e" % (k, err))
+        assert err < 1e-5
+    # the field of the next step is draw 6
+    want = T.htherm(N3, CELL, 3, 6, 300.0, 0.1, 8e5, O.GAMMA0, dt)
+    assert np.abs(s.getarray("H_therm") - want).max() <= 1e-6 * np.abs(want).max()
+    s.close()
+
+
+def test_temperature_changes_the_trajectory_only_through_the_field(mmx_gpu):
+    """Temp = 0 with the module loaded follows the athermal solver exactly."""
+    _say("Temp 0")
+    a = _sim(mmx_gpu, maps=False, temp=0.0)
+    b = mmx_gpu.Sim(N3, CELL)
+    b.setv("Msat", 8e5)
+    b.setv("Aex", 1.3e-11)
+    b.setv("alpha", 0.1)
+    m = O.normalize(np.random.default_rng(8).normal(size=(3,) + N3)).astype(np.float32)
+    for s in (a, b):
+        s.setarray("m", m)
+        s.steps(5)
+    assert np.array_equal(a.getarray("m"), b.getarray("m"))
+    a.close()
+    b.close()
