@@ -58,7 +58,7 @@ class Sim:
 
     def getv(self, name, ncomp=None):
         if ncomp is None:
-            ncomp = 3 if name in ("<m>", "B_ext", "anisU", "j") else 1
+            ncomp = 3 if name in ("<m>", "<H_therm>", "B_ext", "anisU", "j") else 1
         buf = (C.c_double * ncomp)()
         check(lib().mmx_sim_getv(self._h, name.encode(), buf, ncomp))
         return list(buf) if ncomp > 1 else buf[0]

@@ -119,6 +119,51 @@ def test_sp4_am12_against_standard_solution(api):
     assert late < 0.35
 
 
+def test_film_config_full_run():
+    """The film configuration as specified: 128x32x1, seeded random m, 10 mT
+    in-plane field, adaptive rk12 at m_maxerror 1e-5, run 1 ns recording <m>
+    every 10 ps and the final H_eff.  From a random state the trajectory is
+    chaotic (a 1e-7 perturbation changes <m> by O(0.1) within 1 ns), so it is
+    checked through what the dynamics fix: the total energy never increases
+    (alpha > 0, static field: dE/dt = -alpha ... |m x H|^2 <= 0), |m| = 1, and
+    the final H_eff equals the FP64 oracle's at the final m (1e-5)."""
+    import mmx
+    from mmx import synth
+    from oracle import oracle as O
+    cfg = synth.config(1)
+    _say("film config, 1 ns")
+    sim = mmx.Sim(cfg["n"], cfg["cell"])
+    synth.setup_sim(sim, cfg)
+    trace, energy = [], []
+    t_next = 0.0
+    while True:
+        t = sim.getv("t")
+        if t >= t_next:
+            trace.append([t] + list(sim.getv("<m>", 3)))
+            energy.append(sim.getv("E"))
+            t_next += 10e-12
+        if t >= cfg["run"]:
+            break
+        sim.step()
+    trace = np.array(trace)
+    energy = np.array(energy)
+    _say("steps %d, samples %d, E %.4e -> %.4e J" % (sim.getv("step"), len(trace), energy[0],
+                                                      energy[-1]))
+    assert len(trace) >= 100
+    rise = np.diff(energy)
+    assert rise.max() <= 1e-6 * np.abs(energy).max(), rise.max()
+    assert energy[-1] < energy[0]
+    m = sim.getarray("m")
+    assert np.abs(np.sqrt((m.astype(np.float64) ** 2).sum(axis=0)) - 1).max() < 1e-6
+    h = sim.getarray("H_eff")
+    prob = O.Problem(cfg["n"], cfg["cell"], msat=cfg["msat"], aex=cfg["aex"],
+                     alpha=cfg["alpha"], bext=cfg["bext"])
+    want = prob.heff(m.astype(np.float64))
+    err = np.abs(h - want).max() / np.abs(want).max()
+    _say("final H_eff rel err %.2e" % err)
+    assert err < 1e-5
+
+
 def test_sp5_against_reference_trace(api):
     """examples/stdproblem5.py: vortex relaxed at alpha = 1 (3 ns: the
     reference trace starts from a fully centred vortex, which the script's
