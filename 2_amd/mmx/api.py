@@ -27,10 +27,20 @@ def makearray(ncomp, nx, ny, nz):
     return np.zeros((ncomp, nx, ny, nz)).tolist()
 
 
+def resample(arr, size):
+    """engine/resample.go:22-42: out[c][i][j][k] = in[c][i*n1/n2][j*..][k*..]
+    (integer division) for an array [c][x][y][z] onto size (nx, ny, nz)."""
+    a = np.asarray(arr, dtype=np.float64)
+    if a.shape[1:] == tuple(size):
+        return a
+    ix, iy, iz = (np.arange(n2) * n1 // n2 for n1, n2 in zip(a.shape[1:], size))
+    return a[:, ix][:, :, iy][:, :, :, iz]
+
+
 class Engine:
     """One mumax/2 engine restricted to this path."""
 
-    VECTORS = ("B_ext", "anisU", "j", "<m>")
+    VECTORS = ("B_ext", "anisU", "j", "<m>", "<H_therm>")
 
     UNITS = {"m": "", "H_eff": "A/m", "B": "T", "torque": "/s"}   # modules/*.go AddNewQuant
 
@@ -90,12 +100,15 @@ class Engine:
     def gets(self, name):
         return self.getv(name)
 
+    def _resampled(self, arr):
+        """[c][x][y][z] onto the grid size by nearest (floored) cells, as
+        API.SetArray/SetMask resample an array of another size
+        (engine/api.go:234-264, engine/resample.go:22-42)."""
+        return resample(arr, self.size)
+
     def _to_internal(self, arr):
         """[3][nx][ny][nz] (user) -> (3, nz, ny, nx) internal order, comps swapped."""
-        a = np.asarray(arr, dtype=np.float64)
-        nx, ny, nz = self.size
-        if a.shape[1:] == (1, 1, 1):
-            a = np.broadcast_to(a, (a.shape[0], nx, ny, nz))
+        a = self._resampled(arr)
         a = a[::-1].transpose(0, 3, 2, 1)
         return np.ascontiguousarray(a, dtype=np.float32)
 
@@ -111,10 +124,13 @@ class Engine:
     def setmask(self, name, arr):
         s = self._need()
         a = np.asarray(arr, dtype=np.float64)
-        if a.ndim == 4 and a.shape[0] == 3:
+        if a.ndim == 3:
+            a = a[None]
+        if a.shape[0] == 3:
             s.setmask(name, self._to_internal(a), 3)
         else:
-            s.setmask(name, np.ascontiguousarray(np.asarray(a, np.float32).transpose(2, 1, 0)), 1)
+            a = self._resampled(a)[0]
+            s.setmask(name, np.ascontiguousarray(a.astype(np.float32).transpose(2, 1, 0)), 1)
 
     # -- output (engine.go:98-105,554-582, notify_autosave.go:25-31) -----
     def outputdir(self, d):

@@ -147,3 +147,39 @@ def test_temperature_changes_the_trajectory_only_through_the_field(mmx_gpu):
     assert np.array_equal(a.getarray("m"), b.getarray("m"))
     a.close()
     b.close()
+
+
+def test_reference_temperature_script(mmx_gpu):
+    """tests/temperature.py of the reference through the user API: 32x32x1,
+    micromagnetism + solver/rk12 + temperature/brown, Temp 100 K masked to
+    the upper half in y (a 1x2x1 mask, resampled), 1 ns.  The field is zero
+    where the mask is zero, and elsewhere its spread is the Brown variance at
+    the current dt."""
+    from mmx import api
+    _say("reference temperature.py")
+    e = api.reset()
+    e.setgridsize(32, 32, 1)
+    e.setcellsize(5e-9, 5e-9, 5e-9)
+    e.load("micromagnetism")
+    e.load("solver/rk12")
+    e.load("temperature/brown")
+    e.setv("Msat", 800e3)
+    e.setv("Aex", 1.3e-11)
+    e.setv("alpha", 0.01)
+    e.setv("dt", 1e-12)
+    e.setv("m_maxerror", 1.0 / 100)
+    e.setv("temp", 100)
+    e.setmask("temp", [[[[0], [1]]]])
+    e.setarray("m", [[[[1]]], [[[1]]], [[[0]]]])
+    tab = e.autotabulate(["t", "<m>"], period=10e-12)
+    e.run(1e-9)
+    assert len(tab.rows) >= 100
+    h = e.getarray("H_therm")                      # user order [c][x][y][z]
+    assert not np.any(h[:, :, :16, :])
+    hot = h[:, :, 16:, :].ravel()
+    dt = e.getv("dt")
+    sigma = np.sqrt(2 * 0.01 * T.KB * 100 / (T.MU0 * 125e-27 * O.GAMMA0 * dt * 800e3))
+    _say("steps %d, dt %.3e, std %.4e vs %.4e, <m> %s" % (e.getv("step"), dt, hot.std(), sigma,
+                                                        np.round(e.getv("<m>"), 4)))
+    assert abs(hot.std() / sigma - 1) < 0.05
+    assert abs(hot.mean()) < 0.1 * sigma
