@@ -345,6 +345,24 @@ def test_rk12_free_running_cfg1(torch_gpu):
         assert np.abs(g - o).max() < 2e-4, (t_check, g, o)
 
 
+def test_maxtorque_is_the_max_norm(torch_gpu):
+    """tests/reduce.py: maxtorque (MaxNormUpdater of the torque) equals the
+    largest |torque| over the cells, and is at least the largest component."""
+    import mmx
+    n, cell = (1, 32, 32), (3e-9, 500e-9 / 32 / 4, 500e-9 / 32 / 4)
+    _say("maxtorque")
+    sim = mmx.Sim(n, cell)
+    sim.setv("Msat", 800e3)
+    sim.setv("Aex", 1.3e-11)
+    sim.setv("alpha", 1.0)
+    sim.setarray("m", O.normalize(np.random.default_rng(9).normal(size=(3,) + n)).astype(np.float32))
+    mt = sim.getv("maxtorque")
+    tq = sim.getarray("torque").astype(np.float64) * sim.getv("gamma")
+    norm = np.sqrt((tq ** 2).sum(axis=0)).max()
+    assert abs(mt - norm) <= 1e-6 * norm
+    assert mt >= np.abs(tq).max()
+
+
 def test_step_counts_and_errors(torch_gpu):
     import mmx
     sim = mmx.Sim((1, 8, 8), (3e-9, 5e-9, 5e-9))

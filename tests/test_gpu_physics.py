@@ -34,17 +34,21 @@ def api():
     return A
 
 
-def test_sp4_against_standard_solution(api):
-    """tests/stdproblem4.py: relax (1,1,0) at alpha = 1 for 2 ns, then
-    B = (-24.6, 4.3, 0) mT at alpha = 0.02; <m>(t) every 10 ps for 1 ns against
-    tests/stdsolution4.txt.  Published SP4 solutions agree with each other to a
-    few percent until the magnetisation reverses (~0.15-0.2 ns) and drift apart
-    afterwards, so the check is tight early and loose late."""
-    _say("SP4")
+@pytest.mark.parametrize("grid", [(128, 32, 1), (128, 64, 1), (128, 32, 2)])
+def test_sp4_against_standard_solution(api, grid):
+    """tests/stdproblem4.py (128x32x1), stdproblem4_nonsquare.py (128x64x1:
+    cells 3.9 x 1.95 nm) and stdproblem4_3D.py (128x32x2: two 1.5 nm layers):
+    relax (1,1,0) at alpha = 1 for 2 ns, then B = (-24.6, 4.3, 0) mT at alpha =
+    0.02; <m>(t) every 10 ps for 1 ns against tests/stdsolution4.txt.
+    Published SP4 solutions agree with each other to a few percent until the
+    magnetisation reverses (~0.15-0.2 ns) and drift apart afterwards, so the
+    check is tight early and loose late."""
+    _say("SP4", grid)
     ref = np.loadtxt(os.path.join(GOLDEN, "stdsolution4.txt"))
     e = api.reset()
-    e.setgridsize(128, 32, 1)
-    e.setcellsize(500e-9 / 128, 125e-9 / 32, 3e-9)
+    nx, ny, nz = grid
+    e.setgridsize(nx, ny, nz)
+    e.setcellsize(500e-9 / nx, 125e-9 / ny, 3e-9 / nz)
     e.load("micromagnetism")
     e.setv("Msat", 800e3)
     e.setv("Aex", 1.3e-11)
