@@ -51,6 +51,11 @@ struct mmx_sim {
     mmx_reductor *red = nullptr;
     float *m = nullptr, *mnew = nullptr, *mpred = nullptr, *tau0 = nullptr, *heff = nullptr,
           *torq = nullptr, *bf = nullptr;
+    // solver/rk12: the next step's first stage, computed at the corrected m
+    // while the host runs the step controller (valid until the state changes)
+    float *tau0_spec = nullptr;
+    bool spec_valid = false;
+    hipEvent_t ev_err = nullptr;
     unsigned int *derr = nullptr; // [0]: RK error; [4 .. 12]: implicit-solver maxima
     unsigned int *herr = nullptr; // pinned, same layout
     hipStream_t st = nullptr;

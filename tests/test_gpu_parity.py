@@ -363,6 +363,31 @@ def test_maxtorque_is_the_max_norm(torch_gpu):
     assert mt >= np.abs(tq).max()
 
 
+def test_rk12_look_ahead_changes_nothing(torch_gpu):
+    """solver/rk12 computes the next step's first stage at the corrected m while
+    the host decides on the step; a run where that look-ahead is discarded
+    before every step (device_array invalidates it) must be bit-identical."""
+    import mmx
+    from mmx import synth
+    cfg = synth.config(1)
+    _say("rk12 look-ahead")
+    runs = []
+    for defeat in (False, True):
+        sim = mmx.Sim(cfg["n"], cfg["cell"])
+        synth.setup_sim(sim, cfg)
+        for _ in range(300):
+            if defeat:
+                sim.device_array("m")
+            sim.step()
+        runs.append((sim.getarray("m"), sim.getv("t"), sim.getv("dt"), sim.getv("badsteps"),
+                     sim.getv("evals")))
+        sim.close()
+    (ma, ta, da, ba, ea), (mb, tb, db, bb, eb) = runs
+    assert np.array_equal(ma, mb) and ta == tb and da == db and ba == bb
+    _say("evals with look-ahead %d, without %d, badsteps %d" % (ea, eb, ba))
+    assert ea <= eb + ba + 1
+
+
 def test_step_counts_and_errors(torch_gpu):
     import mmx
     sim = mmx.Sim((1, 8, 8), (3e-9, 5e-9, 5e-9))
