@@ -372,6 +372,7 @@ int load_sims(BatchImpl &B, double duration, long long nsteps)
                     "batch: simulation %d has another solver than solver/rk12 loaded", s);
         MMX_REQUIRE(!x->thermal, MMX_ERR_ARG,
                     "batch: simulation %d has temperature/brown loaded", s);
+        MMX_REQUIRE(!x->dex, MMX_ERR_ARG, "batch: simulation %d has demagexch loaded", s);
         x->spec_valid = false; // the batch advances m, t, dt itself
         MMX_TRY(sim_prepare(x));
         MMX_HIP(hipStreamSynchronize(x->st));

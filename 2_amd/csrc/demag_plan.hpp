@@ -86,6 +86,9 @@ struct DemagPlanImpl {
     bool fast_x = false, fast_y = false, fast_z = false; // register-FFT passes
     hipEvent_t *events = nullptr; // when set: recorded before/after every pass
     int nevent = 0;
+    // demagexch kernel build: real-space kernel Ms K_d + dex_cex K_ex
+    bool dex = false;
+    double dex_ms = 0, dex_cex = 0;
 };
 
 // Pass names in launch order (3-D grids; films run xf, y2d, xi).
@@ -93,6 +96,11 @@ constexpr int kMaxPasses = 5;
 
 // Kernel construction (kernel_build.hip)
 int build_demag_kernel(DemagPlanImpl *P, bool keep_real);
+// The demagexch octant (Kernel_Arne at `accuracy` combined with the exchange
+// kernel, transformed like the demag kernel) into out (6 x octant floats),
+// and out = kd + kx / ms over n floats; both on the null stream, synchronised.
+int build_dex_kernel(const DemagPlanImpl *P, int accuracy, double ms, double aex, float *out);
+int fold_kernels(const float *kd, const float *kx, double ms, float *out, long n);
 
 // Launch the five (3-D) or three (2-D) passes; the last one runs `mode`.
 int demag_forward(DemagPlanImpl *P, const float *m, const float *msat_map,

@@ -379,6 +379,63 @@ __global__ void __launch_bounds__(kBlock) extract_octant(const double2 *__restri
     block_max_to(good, maxgood);
 }
 
+// ------------------------------------------------------------ demagexch
+//
+// Exch6NgbrKernel (modules/kernel_exchange.go:102-124): the 6-neighbour
+// Laplacian on the padded grid, the same for the three diagonal components,
+// its seven float32 values assigned one after another at wrapped positions
+// (kernel.go:19-27): a later value replaces an earlier one at the same
+// position (on a length-1 axis the centre ends up 1/h_x^2).  dexKernUpdater
+// (mod_demagexchange.go:134-149): dex = float32(Ms K_d + cex K_ex) with
+// cex = 2 Aex / (mu0 Ms), from the float32 Arne kernel.
+
+struct ExchPoints {
+    long off[7];
+    float val[7];
+    int n;
+};
+
+static ExchPoints exch6_points(const int p[3], const double cell[3])
+{
+    const double h[3] = {cell[0] * cell[0], cell[1] * cell[1], cell[2] * cell[2]};
+    const int d[7][3] = {{0, 0, 0}, {1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1}};
+    const float v[7] = {(float)(-2 / h[0] - 2 / h[1] - 2 / h[2]), (float)(1 / h[0]), (float)(1 / h[0]),
+                        (float)(1 / h[1]), (float)(1 / h[1]), (float)(1 / h[2]), (float)(1 / h[2])};
+    ExchPoints e{};
+    for (int t = 0; t < 7; t++) {
+        long off = 0;
+        for (int a = 0; a < 3; a++) off = off * p[a] + ((d[t][a] % p[a]) + p[a]) % p[a];
+        int k = 0;
+        while (k < e.n && e.off[k] != off) k++;
+        if (k == e.n) e.n++;
+        e.off[k] = off;
+        e.val[k] = v[t];
+    }
+    return e;
+}
+
+__global__ void __launch_bounds__(kBlock) dex_combine(float *__restrict__ K6, long Pn, double ms,
+                                                      double cex, ExchPoints e)
+{
+    const long n = 6 * Pn;
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        const long c = i / Pn, o = i - c * Pn;
+        double v = ms * (double)K6[i];
+        if (c < 3)
+            for (int k = 0; k < e.n; k++)
+                if (e.off[k] == o) v += cex * (double)e.val[k];
+        K6[i] = (float)v;
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) fold_kernel(const float *__restrict__ kd,
+                                                      const float *__restrict__ kx, double inv_ms,
+                                                      float *__restrict__ out, long n)
+{
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        out[i] = (float)((double)kd[i] + (double)kx[i] * inv_ms);
+}
+
 static int make_line_fft_f64(int L, LineFFT<double> &f, std::vector<double2> &host)
 {
     f.L = L;
@@ -484,6 +541,11 @@ int build_demag_kernel(DemagPlanImpl *P, bool keep_real)
                 if (hipGetLastError() != hipSuccess) { set_error("arne_kernel launch failed"); fail(MMX_ERR_HIP); break; }
             }
             if (rc) break;
+            if (P->dex) {
+                dex_combine<<<dim3(4096), dim3(kBlock)>>>(K6, Pn, P->dex_ms, P->dex_cex,
+                                                          exch6_points(p, P->cell));
+                if (hipGetLastError() != hipSuccess) { set_error("dex_combine launch failed"); fail(MMX_ERR_HIP); break; }
+            }
             // IsZero test per component (maxwell_convolution.go:412-418)
             for (int c = 0; c < 6; c++)
                 maxabs_kernel<<<dim3(gb), dim3(kBlock)>>>(K6 + c * Pn, Pn, stats + c);
@@ -526,7 +588,9 @@ int build_demag_kernel(DemagPlanImpl *P, bool keep_real)
             // interleaved storage: component c of point o at P->kern[6*o + c]
             // (6*octant floats); extract_octant / zero_strided write out[6*t]
             float *dst = P->kern + c;
-            if (kmax == 0.0f || cmax[c] / kmax < 1e-5f) {
+            // (ConvPlan.loadKernel, used by demagexch, transforms every
+            // component: convplan.go:85-115)
+            if (!P->dex && (kmax == 0.0f || cmax[c] / kmax < 1e-5f)) {
                 zero_strided<<<dim3(1024), dim3(kBlock)>>>(dst, octant);
                 if (hipGetLastError() != hipSuccess) { set_error("kernel zero launch failed"); fail(MMX_ERR_HIP); break; }
                 continue;
@@ -574,6 +638,29 @@ int build_demag_kernel(DemagPlanImpl *P, bool keep_real)
         return MMX_ERR_HIP;
     }
     return rc;
+}
+
+int build_dex_kernel(const DemagPlanImpl *P, int accuracy, double ms, double aex, float *out)
+{
+    MMX_REQUIRE(P && out && accuracy > 0 && ms != 0.0, MMX_ERR_ARG, "build_dex_kernel: bad arguments");
+    DemagPlanImpl Q = *P;   // geometry, cells, periodicity; nothing of P is written
+    Q.kern = out;
+    Q.real_kernel = nullptr;
+    Q.accuracy = accuracy;
+    Q.cell_kernel_type = MMX_KERNEL_ARNE;
+    Q.dex = true;
+    Q.dex_ms = ms;
+    Q.dex_cex = (2.0 * aex) / (kMu0 * ms);
+    return build_demag_kernel(&Q, false);
+}
+
+int fold_kernels(const float *kd, const float *kx, double ms, float *out, long n)
+{
+    MMX_REQUIRE(kd && kx && out && n > 0 && ms != 0.0, MMX_ERR_ARG, "fold_kernels: bad arguments");
+    fold_kernel<<<dim3(1024), dim3(kBlock)>>>(kd, kx, 1.0 / ms, out, n);
+    MMX_CHECK_LAUNCH();
+    MMX_HIP(hipDeviceSynchronize());
+    return MMX_OK;
 }
 
 } // namespace mmx

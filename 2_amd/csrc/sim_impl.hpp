@@ -38,6 +38,15 @@ struct mmx_sim {
     // temperature/brown (mod_temperature.go:22-134): Temp multiplier and map,
     // Therm_seed, cutoff_dt; the field and the state of its updater
     bool thermal = false;
+    // demagexch (mod_demagexchange.go): H_dex = conv(m, Ms K_d + 2 Aex/(mu0 Ms) K_ex)
+    // with K_d the Arne kernel at demag_acc; the H_eff evaluations convolve with
+    // kern_fold = demag octant + kern_dex / Ms (one convolution for both fields)
+    bool dex = false;
+    int dex_acc = 8;
+    float *kern_dex = nullptr, *kern_fold = nullptr;
+    bool dex_valid = false;
+    double dex_ms = 0, dex_aex = 0;
+    int dex_built_acc = 0;
     double temp = 0, therm_seed = 0, cutoff_dt = 0;
     float *temp_map = nullptr, *htherm = nullptr;
     unsigned long long therm_draws = 0; // draws since the seed was set

@@ -218,7 +218,11 @@ int mmx_sim_destroy(mmx_sim *sim);
  * mod_temperature.go:22-134; see mmx_philox4x32_10 for the noise),
  * "solver/am12" (adaptive implicit-Euler predictor / trapezoidal corrector;
  * solve_am01.go:45-288; loading it sets mindt = 1e-38, maxdt = 1e38 as the
- * module does), which replace each other ("micromagnetism", "demag",
+ * module does), which replace each other, and "demagexch" (H_dex =
+ * conv(m, Ms K_d + 2 Aex / (mu0 Ms) K_ex) added to H_eff, K_d Kernel_Arne at
+ * demag_acc, K_ex the 6-neighbour Laplacian; mod_demagexchange.go:26-205; the
+ * H_eff evaluations fold it into the demag convolution, so it costs no pass;
+ * an Msat mask, the surrogate and batches are refused with it) ("micromagnetism", "demag",
  * "exchange6", "zeeman", "llg", "maxtorque" are already loaded).  Names are
  * case-insensitive, as the reference's quantity names are. */
 int mmx_sim_load(mmx_sim *sim, const char *module);
@@ -229,7 +233,8 @@ int mmx_sim_set_kernel(mmx_sim *sim, int kernel_type, int accuracy);
  * Ku, anisU[3], j[3], polarisation, xi, t, dt, mindt, maxdt, m_maxerror,
  * m_maxIterations and m_maxIterError (bdf-euler; default 3, 1e-5),
  * m_maxAbsError and m_maxRelError (am12; default 1e-4, 1e-3), Temp,
- * Therm_seed and cutoff_dt (temperature/brown);
+ * Therm_seed and cutoff_dt (temperature/brown), demag_acc (demagexch; a
+ * positive integer, default 8);
  * read-only: step, badsteps (am12: of the last step), bdf_iterations (of the
  * last step), m_error, m_peakerror, <m>[3], <H_therm>[3], maxtorque,
  * evals (H_eff evaluations so far), and the energies of micromag/energy
@@ -242,7 +247,7 @@ int mmx_sim_getv(mmx_sim *sim, const char *name, double *v, int ncomp);
 int mmx_sim_setmask(mmx_sim *sim, const char *name, const float *host,
                     int ncomp);
 /* setarray/getarray (host float32, SoA, internal order): m (set/get),
- * H_eff, torque, B, H_therm (get; updated first, like Quant.Update). */
+ * H_eff, torque, B, H_therm, H_dex (get; updated first, like Quant.Update). */
 int mmx_sim_setarray(mmx_sim *sim, const char *name, const float *host,
                      int ncomp);
 int mmx_sim_getarray(mmx_sim *sim, const char *name, float *host, int ncomp);
