@@ -528,6 +528,70 @@ def run_surrogate(args):
     }
 
 
+def run_demagexch(args):
+    """tests/fftbench.py of the reference: 1024x1024x1 cells of 5 nm,
+    micromagnetism + demagexch, Msat 800e3, Aex 1.3e-11, m = (1, 1, 0); H_dex
+    forced to update 100 times (debug_invalidate + debug_update).  Also H_eff
+    with and without demagexch loaded (the module is folded into the demag
+    convolution, so both should take the same time)."""
+    import numpy as np
+    import torch
+    import mmx
+
+    def say(*a):
+        print("[demagexch]", *a, file=sys.stderr, flush=True)
+
+    n, cell, reps = (1, 1024, 1024), (5e-9, 5e-9, 5e-9), 100
+    m = np.zeros((3,) + n, np.float32)
+    m[1] = m[2] = 1.0 / np.sqrt(2.0)     # user (1, 1, 0) in internal (z, y, x) order
+
+    def make(dex):
+        s = mmx.Sim(n, cell)
+        s.setv("Msat", 800e3)
+        s.setv("Aex", 1.3e-11)
+        s.setv("alpha", 1.0)
+        s.setv("dt", 1e-12)
+        if dex:
+            s.load("demagexch")
+        s.setarray("m", m)
+        return s
+
+    def timed(s, name):
+        for _ in range(3):
+            s.update(name)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            s.update(name)
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / reps
+
+    t0 = time.perf_counter()
+    sd = make(True)
+    sd.update("H_dex")
+    torch.cuda.synchronize()
+    t_build = time.perf_counter() - t0
+    t_hdex = timed(sd, "H_dex")
+    t_heff_dex = timed(sd, "H_eff")
+    sd.close()
+    sp = make(False)
+    t_heff = timed(sp, "H_eff")
+    sp.close()
+    say("H_dex %.1f us, H_eff with demagexch %.1f us, without %.1f us" %
+        (1e6 * t_hdex, 1e6 * t_heff_dex, 1e6 * t_heff))
+    return {
+        "metric": "H_dex updates/s (tests/fftbench.py: 1024x1024x1 demagexch, 100 updates)",
+        "value": 1.0 / t_hdex, "unit": "H_dex updates/s", "n_gpus": 1, "steps": reps,
+        "warmup": 3, "ms_per_step": 1e3 * t_hdex, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "demagexch film 1024x1024x1 cells of 5 nm (padded 1x2048x2048), "
+                               "Msat 800e3, Aex 1.3e-11, m = (1, 1, 0)",
+                   "steps_are": "H_dex updates (one convolution with the combined kernel)"},
+        "kernels_build_s": t_build,
+        "heff_ms_with_demagexch": 1e3 * t_heff_dex, "heff_ms_without": 1e3 * t_heff,
+    }
+
+
 def main():
     if len(sys.argv) > 1 and sys.argv[1] == "--summarize":
         print(summarize_profiles(*sys.argv[2:5]))
@@ -538,7 +602,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--rk-steps", type=int, default=1000)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=("bulk", "precision", "batch", "surrogate"),
+    ap.add_argument("--workload", choices=("bulk", "precision", "batch", "surrogate", "demagexch"),
                     default="bulk",
                     help="bulk: 256x256x64 (the headline line); precision: 1024x1024x32 film "
                          "with the FP64 Newell kernel and per-field errors against the FP64 "
@@ -571,6 +635,9 @@ def main():
         return
     if args.workload == "surrogate":
         print(json.dumps(run_surrogate(args)))
+        return
+    if args.workload == "demagexch":
+        print(json.dumps(run_demagexch(args)))
         return
     cfg = synth.config(2)
     sim = mmx.Sim(cfg["n"], cfg["cell"])

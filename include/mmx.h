@@ -255,7 +255,8 @@ int mmx_sim_getarray(mmx_sim *sim, const char *name, float *host, int ncomp);
 int mmx_sim_device_array(mmx_sim *sim, const char *name, float **dev);
 /* The stream all of the sim's work is queued on (hipStream_t). */
 int mmx_sim_stream(mmx_sim *sim, void **stream);
-/* Quant.Update of H_eff (one effective-field evaluation), torque or B.
+/* Quant.Update of H_eff (one effective-field evaluation), torque, B or
+ * H_dex (demagexch; computed into the storage B's update uses).
  * Asynchronous: queues the kernels and returns. */
 int mmx_sim_update(mmx_sim *sim, const char *name);
 /* Engine.Step / Steps / Run with the loaded solver.  A solver that cannot
