@@ -1,7 +1,7 @@
 """demagexch on the GPU, through the C ABI, against oracle/demagexch.py.
 
 * H_dex (getarray) against the FP64 oracle conv(m, dex) on 3-D, film, periodic
-  and non-power-of-two grids, to 1e-5 of max |H_dex|; with Aex = 0 the field
+  (in y, in z) and non-power-of-two grids, to 1e-5 of max |H_dex|; with Aex = 0 the field
   is the demag field of the accuracy-8 kernel alone, checked to the same bar;
 * H_eff with demagexch loaded (micromagnetism's terms plus H_dex, evaluated as
   one convolution with the folded kernel) against the oracle Problem given the
@@ -53,7 +53,8 @@ def _sim(mmx, n, cell, pbc=(0, 0, 0), msat=8e5, aex=1.3e-11, seed=3):
 HDEX_CASES = [((4, 16, 32), (2e-9, 3e-9, 5e-9), (0, 0, 0)),
               ((1, 32, 64), (3e-9, 3.90625e-9, 3.90625e-9), (0, 0, 0)),
               ((4, 16, 32), (3e-9, 3e-9, 3e-9), (0, 1, 0)),
-              ((3, 10, 12), (2e-9, 3e-9, 4e-9), (0, 0, 0))]
+              ((3, 10, 12), (2e-9, 3e-9, 4e-9), (0, 0, 0)),
+              ((8, 8, 16), (3e-9, 2e-9, 2.5e-9), (1, 0, 0))]
 
 
 @pytest.mark.parametrize("aex", [1.3e-11, 0.0])
