@@ -94,6 +94,71 @@ exchange6_kernel(float *__restrict__ hx, float *__restrict__ hy, float *__restri
     hz[I] = H[2];
 }
 
+// exchange6 without maps, its stencil staged in LDS: a workgroup takes a
+// kEJ x kEK tile of (j, k) in one i-plane and stages m there with a one-cell
+// halo, each LDS position holding the cell the neighbour rule names (clamped
+// at open edges, wrapped at periodic ones), so the j and k neighbours come
+// from LDS and only i - 1 and i + 1 from global memory.  Per cell the
+// arithmetic is that of exchange6_kernel with unit weights.
+constexpr int kEJ = 4, kEK = 64;
+static_assert(kEJ * kEK == kBlock, "one thread per tile cell");
+
+__device__ inline int nb_index(int x, int len, int wrap)
+{
+    if (x < 0) x = wrap ? x + len : 0;
+    else if (x >= len) x = wrap ? x - len : len - 1;
+    return min(max(x, 0), len - 1);   // positions past a partial tile: any cell
+}
+
+__global__ void __launch_bounds__(kBlock)
+exchange6_lds_kernel(float *__restrict__ hx, float *__restrict__ hy, float *__restrict__ hz,
+                     const float *__restrict__ mx, const float *__restrict__ my,
+                     const float *__restrict__ mz, float pre, int N0, int N1, int N2, int w0,
+                     int w1, int w2, float c0, float c1, float c2, int nj, int nk)
+{
+    __shared__ float tile[3][kEJ + 2][kEK + 2];
+    const int b = blockIdx.x;
+    const int k0 = (b % nk) * kEK, j0 = ((b / nk) % nj) * kEJ, i = b / (nk * nj);
+    const long plane = (long)N1 * N2;
+    const float *mm[3] = {mx, my, mz};
+    for (int q = threadIdx.x; q < (kEJ + 2) * (kEK + 2); q += kBlock) {
+        const int jj = q / (kEK + 2), kk = q % (kEK + 2);
+        const long src = (long)i * plane + (long)nb_index(j0 + jj - 1, N1, w1) * N2 +
+                         nb_index(k0 + kk - 1, N2, w2);
+#pragma unroll
+        for (int c = 0; c < 3; c++) tile[c][jj][kk] = mm[c][src];
+    }
+    __syncthreads();
+    const int tj = threadIdx.x / kEK, tk = threadIdx.x % kEK;
+    const int j = j0 + tj, k = k0 + tk;
+    if (j >= N1 || k >= N2) return;
+    const long I = (long)i * plane + (long)j * N2 + k;
+    const long A0 = (long)nb_index(i - 1, N0, w0) * plane + (long)j * N2 + k;
+    const long B0 = (long)nb_index(i + 1, N0, w0) * plane + (long)j * N2 + k;
+    const float l1 = 1.0f, l2 = 1.0f;
+    float H[3] = {0.0f, 0.0f, 0.0f};
+    float m0[3], a[3], bb[3];
+    // dimension order 0, 2, 1 as in exchange6_kernel
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        m0[c] = tile[c][tj + 1][tk + 1];
+        a[c] = mm[c][A0];
+        bb[c] = mm[c][B0];
+    }
+    const float p0 = pre * c0, p2 = pre * c2, p1 = pre * c1;
+#pragma unroll
+    for (int c = 0; c < 3; c++) H[c] += p0 * (l1 * (a[c] - m0[c]) + l2 * (bb[c] - m0[c]));
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+        H[c] += p2 * (l1 * (tile[c][tj + 1][tk] - m0[c]) + l2 * (tile[c][tj + 1][tk + 2] - m0[c]));
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+        H[c] += p1 * (l1 * (tile[c][tj][tk + 1] - m0[c]) + l2 * (tile[c][tj + 2][tk + 1] - m0[c]));
+    hx[I] = H[0];
+    hy[I] = H[1];
+    hz[I] = H[2];
+}
+
 __global__ void __launch_bounds__(kBlock)
 uniaxial_kernel(float *__restrict__ hx, float *__restrict__ hy, float *__restrict__ hz,
                 const float *__restrict__ mx, const float *__restrict__ my,
@@ -341,9 +406,16 @@ int mmx_exchange6(float *hx, float *hy, float *hz, const float *mx, const float 
     const float c0 = (float)(1.0 / ((double)cell0 * (double)cell0));
     const float c1 = (float)(1.0 / ((double)cell1 * (double)cell1));
     const float c2 = (float)(1.0 / ((double)cell2 * (double)cell2));
-    exchange6_kernel<<<grid1d(N), kBlock, 0, as_stream(stream)>>>(
-        hx, hy, hz, mx, my, mz, msat_map, aex_map, pre, N0, N1, N2, periodic0, periodic1,
-        periodic2, c0, c1, c2);
+    const long nj = div_up(N1, kEJ), nk = div_up(N2, kEK);
+    if (!msat_map && !aex_map && (long)N0 * nj * nk < (1L << 31)) {
+        exchange6_lds_kernel<<<dim3((unsigned)(N0 * nj * nk)), kBlock, 0, as_stream(stream)>>>(
+            hx, hy, hz, mx, my, mz, pre, N0, N1, N2, periodic0, periodic1, periodic2, c0, c1, c2,
+            (int)nj, (int)nk);
+    } else {
+        exchange6_kernel<<<grid1d(N), kBlock, 0, as_stream(stream)>>>(
+            hx, hy, hz, mx, my, mz, msat_map, aex_map, pre, N0, N1, N2, periodic0, periodic1,
+            periodic2, c0, c1, c2);
+    }
     MMX_CHECK_LAUNCH();
     return MMX_OK;
 }

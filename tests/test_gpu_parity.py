@@ -177,6 +177,25 @@ def test_exchange6(torch_gpu, pbc, maps):
     assert relerr(got, want) < TOL
 
 
+@pytest.mark.parametrize("n,pbc", [((5, 9, 17), (0, 0, 0)), ((3, 8, 128), (1, 1, 1)),
+                                   ((1, 6, 70), (0, 1, 0)), ((2, 1, 65), (1, 0, 0)),
+                                   ((4, 7, 1), (0, 0, 1))])
+def test_exchange6_lds_path_is_bit_identical(torch_gpu, n, pbc):
+    """Without maps mmx_exchange6 runs the LDS-staged stencil; with an Aex map
+    of ones it runs the per-cell kernel whose weights are then exactly 1: the
+    two must agree bit for bit (partial tiles, wrapped and clamped edges,
+    length-1 axes)."""
+    from mmx import kernels as K
+    torch = torch_gpu
+    _say("exchange6 LDS vs per-cell", n, pbc)
+    m = O.normalize(np.random.default_rng(13).normal(size=(3,) + n)).astype(np.float32)
+    cell, pre = (3e-9, 4e-9, 5e-9), 2 * 1.3e-11 / (O.MU0 * 8e5)
+    a = K.exchange6(_t(torch, m), pre, cell, pbc).cpu().numpy()
+    b = K.exchange6(_t(torch, m), pre, cell, pbc, None,
+                    _t(torch, np.ones(n, np.float32))).cpu().numpy()
+    assert np.array_equal(a, b)
+
+
 def test_uniaxial_torque_normalize(torch_gpu):
     from mmx import kernels as K
     torch = torch_gpu
