@@ -22,20 +22,11 @@ __device__ __constant__ int kStoredIdx[3][3] = {{0, 5, 4}, {-1, 1, 3}, {-1, -1, 
 struct ArneArgs {
     int p[3];
     double cell[3];
-    int pbc[3];
     int x1, x2, y1, y2, z1, z2;
     double L;
     double accuracy;
     int s;
-    int accumulate; // periodic images: several (x,y,z) map to one element
 };
-
-__device__ inline int wrap_idx(int number, int max)
-{
-    while (number < 0) number += max;
-    while (number >= max) number -= max;
-    return number;
-}
 
 __device__ inline double delta_cells(int d)
 {
@@ -44,22 +35,26 @@ __device__ inline double delta_cells(int d)
     return (double)d;
 }
 
-__global__ void __launch_bounds__(kBlock) arne_kernel(ArneArgs a, float *__restrict__ K6)
+// First displacement in [lo, hi] whose wrapped index is w (w in [0, p)), or
+// hi + 1 when there is none; the others follow at +p.  kernel.go:19-28 wraps
+// x into [0, p) by adding or subtracting p.
+__device__ inline int first_image(int w, int p, int lo)
+{
+    int x = w;
+    while (x - p >= lo) x -= p;
+    return x;
+}
+
+// B (FP64) at displacement (x, y, z) of a unit s-dipole: the surface-charge /
+// volume integration of kernel_magnetostatic.go:94-165, same operations in the
+// same order.
+__device__ inline void arne_point(const ArneArgs &a, int x, int y, int z, double Bf[3])
 {
     const int X = 0, Y = 1, Z = 2;
-    const long nx = a.x2 - a.x1 + 1, ny = a.y2 - a.y1 + 1, nz = a.z2 - a.z1 + 1;
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nx * ny * nz) return;
-    const int z = a.z1 + (int)(t % nz);
-    const int y = a.y1 + (int)((t / nz) % ny);
-    const int x = a.x1 + (int)(t / (nz * ny));
     const double *cell = a.cell;
     const int s = a.s;
     const int u = s, v = (s + 1) % 3, w = (s + 2) % 3;
     double R[3], R2[3], pole[3];
-    const int xw = wrap_idx(x, a.p[X]);
-    const int yw = wrap_idx(y, a.p[Y]);
-    const int zw = wrap_idx(z, a.p[Z]);
     R[X] = (double)x * cell[X];
     R[Y] = (double)y * cell[Y];
     R[Z] = (double)z * cell[Z];
@@ -83,7 +78,7 @@ __global__ void __launch_bounds__(kBlock) arne_kernel(ArneArgs a, float *__restr
     const double pu1 = cell[u] / 2.;
     const double pu2 = -pu1;
 
-    double Bf[3] = {0, 0, 0};
+    Bf[0] = Bf[1] = Bf[2] = 0.0;
     for (int i = 0; i < nv; i++) {
         const double pv = -(cell[v] / 2.) + cell[v] / (double)(2 * nv) + (double)i * (cell[v] / (double)nv);
         pole[v] = pv;
@@ -119,15 +114,38 @@ __global__ void __launch_bounds__(kBlock) arne_kernel(ArneArgs a, float *__restr
             }
         }
     }
-    const long P = (long)a.p[0] * a.p[1] * a.p[2];
-    const long off = ((long)xw * a.p[Y] + yw) * a.p[Z] + zw;
+}
+
+// One thread per padded-grid element (xw, yw, zw): the element is the float32
+// sum, starting from 0, of float32(B) over every displacement (x, y, z) in the
+// loop ranges that wraps onto it, added in the reference's loop order (x, then
+// y, then z ascending; kernel_magnetostatic.go:91-101,168-172).  Without
+// periodic images an element has at most one displacement; with them the
+// element is the same float32 sum the reference forms, with no atomics, so the
+// kernel is deterministic and bit-identical to it.  Every element is written.
+__global__ void __launch_bounds__(kBlock) arne_kernel(ArneArgs a, float *__restrict__ K6)
+{
+    const int X = 0, Y = 1, Z = 2;
+    const long P = (long)a.p[X] * a.p[Y] * a.p[Z];
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= P) return;
+    const int zw = (int)(t % a.p[Z]);
+    const int yw = (int)((t / a.p[Z]) % a.p[Y]);
+    const int xw = (int)(t / ((long)a.p[Z] * a.p[Y]));
+    const int xs = first_image(xw, a.p[X], a.x1);
+    const int ys = first_image(yw, a.p[Y], a.y1);
+    const int zs = first_image(zw, a.p[Z], a.z1);
+    float acc[3] = {0.0f, 0.0f, 0.0f};
+    for (int x = xs; x <= a.x2; x += a.p[X])
+        for (int y = ys; y <= a.y2; y += a.p[Y])
+            for (int z = zs; z <= a.z2; z += a.p[Z]) {
+                double Bf[3];
+                arne_point(a, x, y, z, Bf);
+                for (int dd = 0; dd < 3; dd++) acc[dd] += (float)Bf[dd];
+            }
     for (int dd = 0; dd < 3; dd++) {
-        const int idx = kStoredIdx[s][dd];
-        if (idx < 0) continue;
-        if (a.accumulate)
-            atomicAdd(&K6[idx * P + off], (float)Bf[dd]);
-        else
-            K6[idx * P + off] = (float)Bf[dd];
+        const int idx = kStoredIdx[a.s][dd];
+        if (idx >= 0) K6[idx * P + t] = acc[dd];
     }
 }
 
@@ -519,7 +537,6 @@ int build_demag_kernel(DemagPlanImpl *P, bool keep_real)
             for (int i = 0; i < 3; i++) {
                 a.p[i] = p[i];
                 a.cell[i] = P->cell[i];
-                a.pbc[i] = P->pbc[i];
             }
             a.x1 = -(p[0] - 1) / 2; a.x2 = p[0] / 2 - 1;
             a.y1 = -(p[1] - 1) / 2; a.y2 = p[1] / 2 - 1;
@@ -533,11 +550,9 @@ int build_demag_kernel(DemagPlanImpl *P, bool keep_real)
             if (P->cell[2] < L) L = P->cell[2];
             a.L = L;
             a.accuracy = (double)P->accuracy;
-            a.accumulate = (P->pbc[0] || P->pbc[1] || P->pbc[2]) ? 1 : 0;
-            const long npts = (long)(a.x2 - a.x1 + 1) * (a.y2 - a.y1 + 1) * (a.z2 - a.z1 + 1);
             for (int s = 0; s < 3; s++) {
                 a.s = s;
-                arne_kernel<<<dim3(div_up(npts, kBlock)), dim3(kBlock)>>>(a, K6);
+                arne_kernel<<<dim3(div_up(Pn, kBlock)), dim3(kBlock)>>>(a, K6);
                 if (hipGetLastError() != hipSuccess) { set_error("arne_kernel launch failed"); fail(MMX_ERR_HIP); break; }
             }
             if (rc) break;
