@@ -2,6 +2,7 @@
 // gfx950 kernels: normalize, torque, exchange6, uniaxial anisotropy,
 // Zhang-Li, the madd family and the Reductor.  Each C entry point names the
 // reference function it replaces (include/mmx.h).
+#include <mutex>
 #include <vector>
 
 #include "epilogue.hpp"
@@ -267,6 +268,10 @@ __global__ void __launch_bounds__(kBlock) mul_kernel(float *dst, const float *a,
 // ---------------------------------------------------------------- reductor
 
 enum RedOp : int { RED_SUM = 0, RED_DOT, RED_MAX, RED_MIN, RED_MAXABS, RED_MAXDIFF, RED_MAXNORM };
+static_assert(RED_SUM == MMX_RED_SUM && RED_DOT == MMX_RED_DOT && RED_MAX == MMX_RED_MAX &&
+                  RED_MIN == MMX_RED_MIN && RED_MAXABS == MMX_RED_MAXABS &&
+                  RED_MAXDIFF == MMX_RED_MAXDIFF && RED_MAXNORM == MMX_RED_MAXNORM,
+              "reduction codes of mmx.h");
 
 // One partial per workgroup: FP64 sums (sum, dot) or float extrema.
 template <int OP>
@@ -317,6 +322,85 @@ __global__ void __launch_bounds__(kBlock) reduce_kernel(const float *__restrict_
     }
 }
 
+// The reductor's partial count: at most this many workgroups, each reducing
+// a grid-strided share of the input to one FP64 partial.
+constexpr int kRedMaxBlocks = 2048;
+
+static int red_blocks(long n)
+{
+    return (int)std::min<long>(kRedMaxBlocks, std::max<long>(1, div_up(n, 4L * kBlock)));
+}
+
+static void launch_reduce(int op, const float *a, const float *b, const float *c, long n, int nb,
+                          hipStream_t st, double *part)
+{
+    switch (op) {
+    case RED_SUM: reduce_kernel<RED_SUM><<<nb, kBlock, 0, st>>>(a, b, c, n, part); break;
+    case RED_DOT: reduce_kernel<RED_DOT><<<nb, kBlock, 0, st>>>(a, b, c, n, part); break;
+    case RED_MAX: reduce_kernel<RED_MAX><<<nb, kBlock, 0, st>>>(a, b, c, n, part); break;
+    case RED_MIN: reduce_kernel<RED_MIN><<<nb, kBlock, 0, st>>>(a, b, c, n, part); break;
+    case RED_MAXABS: reduce_kernel<RED_MAXABS><<<nb, kBlock, 0, st>>>(a, b, c, n, part); break;
+    case RED_MAXDIFF: reduce_kernel<RED_MAXDIFF><<<nb, kBlock, 0, st>>>(a, b, c, n, part); break;
+    default: reduce_kernel<RED_MAXNORM><<<nb, kBlock, 0, st>>>(a, b, c, n, part); break;
+    }
+}
+
+// Host finish of the partials: compensated sum (sum, dot) or extremum.
+static double finish_partials(int op, const double *hp, int nb)
+{
+    double res = hp[0];
+    if (op == RED_SUM || op == RED_DOT) {
+        double s = 0.0, comp = 0.0;
+        for (int i = 0; i < nb; i++) {
+            const double y = hp[i] - comp;
+            const double t = s + y;
+            comp = (t - s) - y;
+            s = t;
+        }
+        res = s;
+    } else {
+        for (int i = 1; i < nb; i++)
+            res = (op == RED_MIN) ? std::min(res, hp[i]) : std::max(res, hp[i]);
+    }
+    return res;
+}
+
+// ------------------------------------------------ allocation-free reductions
+// mmx_reduce_noalloc keeps its partials in this file-scope device array: one
+// copy per GPU, resident until the process ends (16 KiB).  On the host the
+// name `scratch` is a stand-in, not a device address: each device's address is
+// looked up once with hipGetSymbolAddress (noalloc_setup) and only that
+// address is handed to kernels and copies.
+static __device__ __align__(256) double scratch[kRedMaxBlocks];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_noalloc_mu;
+static double *g_noalloc_parts[kMaxDevices];
+
+// Looks up this device's copy of `scratch`; `dev` must be the current device.
+int noalloc_setup(int dev)
+{
+    MMX_REQUIRE(dev >= 0 && dev < kMaxDevices, MMX_ERR_ARG,
+                "allocation-free reductions: device %d beyond %d", dev, kMaxDevices);
+    std::lock_guard<std::mutex> lock(g_noalloc_mu);
+    if (g_noalloc_parts[dev]) return MMX_OK;
+    void *p = nullptr;
+    MMX_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    MMX_REQUIRE(p != nullptr, MMX_ERR_HIP, "allocation-free reductions: no device address");
+    g_noalloc_parts[dev] = static_cast<double *>(p);
+    return MMX_OK;
+}
+
+static int noalloc_parts(double **out)
+{
+    int dev = -1;
+    MMX_HIP(hipGetDevice(&dev));
+    MMX_TRY(noalloc_setup(dev)); // first call on a device that mmx_init did not set up
+    std::lock_guard<std::mutex> lock(g_noalloc_mu);
+    *out = g_noalloc_parts[dev];
+    return MMX_OK;
+}
+
 } // namespace mmx
 
 struct mmx_reductor {
@@ -333,35 +417,12 @@ static int reduce_run(mmx_reductor *r, int op, const float *a, const float *b, c
 {
     MMX_REQUIRE(r && a, MMX_ERR_ARG, "reductor: NULL argument");
     MMX_REQUIRE(n > 0 && n <= r->n, MMX_ERR_ARG, "reductor: size %ld outside (0, %ld]", n, r->n);
-    const int nb = (int)std::min<long>(r->blocks, std::max<long>(1, div_up(n, 4L * kBlock)));
-    switch (op) {
-    case RED_SUM: reduce_kernel<RED_SUM><<<nb, kBlock, 0, st>>>(a, b, c, n, r->dpart); break;
-    case RED_DOT: reduce_kernel<RED_DOT><<<nb, kBlock, 0, st>>>(a, b, c, n, r->dpart); break;
-    case RED_MAX: reduce_kernel<RED_MAX><<<nb, kBlock, 0, st>>>(a, b, c, n, r->dpart); break;
-    case RED_MIN: reduce_kernel<RED_MIN><<<nb, kBlock, 0, st>>>(a, b, c, n, r->dpart); break;
-    case RED_MAXABS: reduce_kernel<RED_MAXABS><<<nb, kBlock, 0, st>>>(a, b, c, n, r->dpart); break;
-    case RED_MAXDIFF: reduce_kernel<RED_MAXDIFF><<<nb, kBlock, 0, st>>>(a, b, c, n, r->dpart); break;
-    default: reduce_kernel<RED_MAXNORM><<<nb, kBlock, 0, st>>>(a, b, c, n, r->dpart); break;
-    }
+    const int nb = std::min(r->blocks, red_blocks(n));
+    launch_reduce(op, a, b, c, n, nb, st, r->dpart);
     MMX_CHECK_LAUNCH();
     MMX_HIP(hipMemcpyAsync(r->hpart.data(), r->dpart, sizeof(double) * nb, hipMemcpyDeviceToHost, st));
     MMX_HIP(hipStreamSynchronize(st));
-    // finish on the host: compensated sum of the FP64 partials, or extremum
-    double res = r->hpart[0];
-    if (op == RED_SUM || op == RED_DOT) {
-        double s = 0.0, comp = 0.0;
-        for (int i = 0; i < nb; i++) {
-            const double y = r->hpart[i] - comp;
-            const double t = s + y;
-            comp = (t - s) - y;
-            s = t;
-        }
-        res = s;
-    } else {
-        for (int i = 1; i < nb; i++)
-            res = (op == RED_MIN) ? std::min(res, r->hpart[i]) : std::max(res, r->hpart[i]);
-    }
-    *out = res;
+    *out = finish_partials(op, r->hpart.data(), nb);
     return MMX_OK;
 }
 
@@ -518,7 +579,7 @@ int mmx_reductor_create(mmx_reductor **r, long n)
     MMX_REQUIRE(r && n > 0, MMX_ERR_ARG, "mmx_reductor_create: bad arguments");
     mmx_reductor *x = new mmx_reductor();
     x->n = n;
-    x->blocks = (int)std::min<long>(2048, std::max<long>(1, div_up(n, 4L * kBlock)));
+    x->blocks = red_blocks(n);
     x->hpart.resize(x->blocks);
     if (hipMalloc(&x->dpart, sizeof(double) * x->blocks) != hipSuccess) {
         delete x;
@@ -583,6 +644,35 @@ int mmx_reductor_maxnorm(mmx_reductor *r, const float *ax, const float *ay, cons
     double d = 0;
     MMX_TRY(reduce_run(r, RED_MAXNORM, ax, ay, az, n, as_stream(stream), &d));
     *out = (float)sqrt(d);
+    return MMX_OK;
+}
+
+int mmx_reduce_noalloc(int op, const float *a, const float *b, const float *c, long n,
+                       void *stream, double *out)
+{
+    MMX_REQUIRE(op >= MMX_RED_SUM && op <= MMX_RED_MAXNORM, MMX_ERR_ARG,
+                "mmx_reduce_noalloc: unknown operation %d", op);
+    MMX_REQUIRE(a && out, MMX_ERR_ARG, "mmx_reduce_noalloc: NULL argument");
+    MMX_REQUIRE(b || !(op == MMX_RED_DOT || op == MMX_RED_MAXDIFF || op == MMX_RED_MAXNORM),
+                MMX_ERR_ARG, "mmx_reduce_noalloc: operation %d needs b", op);
+    MMX_REQUIRE(c || op != MMX_RED_MAXNORM, MMX_ERR_ARG, "mmx_reduce_noalloc: maxnorm needs c");
+    MMX_REQUIRE(n > 0 && n <= MMX_NOALLOC_MAX_N, MMX_ERR_SIZE,
+                "mmx_reduce_noalloc: size %ld outside (0, %ld] (no workspace is allocated)", n,
+                (long)MMX_NOALLOC_MAX_N);
+    double *parts = nullptr;
+    MMX_TRY(noalloc_parts(&parts));
+    const int nb = red_blocks(n); // <= kRedMaxBlocks: every partial read is written below
+    hipStream_t st = as_stream(stream);
+    launch_reduce(op, a, b, c, n, nb, st, parts);
+    MMX_CHECK_LAUNCH();
+    double hp[kRedMaxBlocks];
+    MMX_HIP(hipMemcpyAsync(hp, parts, sizeof(double) * nb, hipMemcpyDeviceToHost, st));
+    MMX_HIP(hipStreamSynchronize(st));
+    const double res = finish_partials(op, hp, nb);
+    // the float results of the reductor's max/min/maxabs/maxdiff/maxnorm calls
+    if (op == MMX_RED_SUM || op == MMX_RED_DOT) *out = res;
+    else if (op == MMX_RED_MAXNORM) *out = (double)(float)sqrt(res);
+    else *out = (double)(float)res;
     return MMX_OK;
 }
 

@@ -51,6 +51,10 @@ void set_error(const char *fmt, ...);
 
 inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
 
+// Looks up the current device's copy of the allocation-free reductions'
+// partials array (local.hip); called by mmx_init and on first use.
+int noalloc_setup(int dev);
+
 // Physical constants, src/mumax/common/constants.go:16-26.
 constexpr double kPi = 3.14159265358979323846264338327950288;
 constexpr double kMu0 = 4.0 * kPi * 1e-7;

@@ -199,7 +199,7 @@ int mmx_init(int device)
     MMX_HIP(hipGetDeviceCount(&n));
     MMX_REQUIRE(device >= 0 && device < n, MMX_ERR_ARG, "mmx_init: device %d of %d", device, n);
     MMX_HIP(hipSetDevice(device));
-    return MMX_OK;
+    return noalloc_setup(device);
 }
 
 int mmx_demag_plan_create(mmx_demag_plan **plan, int n0, int n1, int n2, double cell0,

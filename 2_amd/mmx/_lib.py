@@ -39,6 +39,7 @@ _SIGS = {
     "mmx_reductor_maxabs": (_i, [_vp, _fp, _l, _vp, C.POINTER(_f)]),
     "mmx_reductor_maxdiff": (_i, [_vp, _fp, _fp, _l, _vp, C.POINTER(_f)]),
     "mmx_reductor_maxnorm": (_i, [_vp, _fp, _fp, _fp, _l, _vp, C.POINTER(_f)]),
+    "mmx_reduce_noalloc": (_i, [_i, _fp, _fp, _fp, _l, _vp, C.POINTER(_d)]),
     "mmx_demag_plan_create": (_i, [C.POINTER(_vp), _i, _i, _i, _d, _d, _d, _i, _i, _i, _i, _i]),
     "mmx_demag_plan_destroy": (_i, [_vp]),
     "mmx_demag_update_b": (_i, [_vp, _fp, _fp, _fp, _fp, _fp, _fp, _fp, _f, _vp]),

@@ -153,6 +153,29 @@ int mmx_reductor_maxdiff(mmx_reductor *r, const float *a, const float *b,
 int mmx_reductor_maxnorm(mmx_reductor *r, const float *ax, const float *ay,
                          const float *az, long n, void *stream, float *out);
 
+/* Allocation-free reduction.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Same kernels and
+ * result as the mmx_reductor_* call named by `op`, for n in (0,
+ * MMX_NOALLOC_MAX_N] floats (the reference's int sizes); larger n fails with
+ * MMX_ERR_SIZE.  Operands: a (sum, max, min, maxabs), a and b (dot, maxdiff),
+ * a, b, c as the x, y, z components (maxnorm).  The result is returned as a
+ * double: the FP64 sum for sum/dot, the float result for the others.  The
+ * partials go to a 16 KiB device array built into the library (one per GPU,
+ * held until the process ends), so no call allocates or frees; calls that
+ * overlap on one GPU (several host threads or streams) race on it.  The call
+ * synchronises `stream`. */
+#define MMX_RED_SUM 0
+#define MMX_RED_DOT 1
+#define MMX_RED_MAX 2
+#define MMX_RED_MIN 3
+#define MMX_RED_MAXABS 4
+#define MMX_RED_MAXDIFF 5
+#define MMX_RED_MAXNORM 6
+#define MMX_NOALLOC_MAX_N 2147483647L
+int mmx_reduce_noalloc(int op, const float *a, const float *b, const float *c,
+                       long n, void *stream, double *out);
+
 /* ------------------------------------------------------------------------
  * Demag convolution plan: MaxwellPlan (src/mumax/modules/maxwell_convolution.go)
  * with its FFTPlanX (src/mumax/gpu/fftplanX.go) and kernel construction
