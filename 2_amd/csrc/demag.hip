@@ -94,9 +94,16 @@ xi_kernel(DemagGeom g, LineFFT<float> fx, int RB, const float2 *__restrict__ xb,
 
 // ------------------------------------------------------------ host launchers
 
-static inline void mark(DemagPlanImpl *P, hipStream_t st)
+// Pass ids of the timed evaluation: 3-D xf 0, yf 1, z 2, yi 3, xi 4; films
+// xf 0, y2d 1, xi 2.  mark(P, st, pass) ends a launch of `pass` (-1: start).
+enum : int { PASS_START = -1, PASS_XF = 0, PASS_YF = 1, PASS_Y2D = 1, PASS_Z = 2, PASS_YI = 3 };
+
+static inline void mark(DemagPlanImpl *P, hipStream_t st, int pass)
 {
-    if (P->events) (void)hipEventRecord(P->events[P->nevent++], st);
+    if (P->events && P->nevent < kMaxMarks) {
+        P->event_pass[P->nevent] = pass;
+        (void)hipEventRecord(P->events[P->nevent++], st);
+    }
 }
 
 static size_t lds_bytes_x(const DemagPlanImpl *P)
@@ -127,7 +134,7 @@ static int y_and_z_passes(DemagPlanImpl *P, hipStream_t st)
     const int ntx_y = div_up(g.h2, P->tiles.w_y);
     if (g.p0 == 1 && P->fast_y) {
         MMX_TRY(fast_y2d(P, st));
-        mark(P, st);
+        mark(P, st, PASS_Y2D);
         return MMX_OK;
     }
     if (g.p0 == 1) {
@@ -135,28 +142,28 @@ static int y_and_z_passes(DemagPlanImpl *P, hipStream_t st)
         allow_lds(y2d_kernel, b);
         y2d_kernel<<<dim3(ntx_y), dim3(kBlock), b, st>>>(g, P->fy, P->tiles.w_y, P->kern, P->xbuf);
         MMX_CHECK_LAUNCH();
-        mark(P, st);
+        mark(P, st, PASS_Y2D);
         return MMX_OK;
     }
     if (P->fast_y) {
         MMX_TRY(fast_yf(P, st));
-        mark(P, st);
+        mark(P, st, PASS_YF);
     } else {
         const size_t b = lds_bytes_y(P, 1);
         allow_lds(yf_kernel, b);
         yf_kernel<<<dim3(ntx_y, g.n0, 3), dim3(kBlock), b, st>>>(g, P->fy, P->tiles.w_y, P->xbuf,
                                                                    P->ybuf);
         MMX_CHECK_LAUNCH();
-        mark(P, st);
+        mark(P, st, PASS_YF);
     }
     if (P->fast_z) {
         MMX_TRY(fast_z(P, st));
-        mark(P, st);
+        mark(P, st, PASS_Z);
     } else if (z_small_applies(g)) {
         z_small_kernel<<<dim3((unsigned)div_up((long)g.p1 * g.h2, kBlock)), dim3(kBlock), 0, st>>>(
             g, P->kern, P->ybuf);
         MMX_CHECK_LAUNCH();
-        mark(P, st);
+        mark(P, st, PASS_Z);
     } else {
         const int ntx_z = div_up(g.h2, P->tiles.w_z);
         const size_t b = lds_bytes_z(P);
@@ -164,18 +171,18 @@ static int y_and_z_passes(DemagPlanImpl *P, hipStream_t st)
         z_kernel<<<dim3(ntx_z, g.p1), dim3(kBlock), b, st>>>(g, P->fz, P->tiles.w_z, P->kern,
                                                               P->ybuf);
         MMX_CHECK_LAUNCH();
-        mark(P, st);
+        mark(P, st, PASS_Z);
     }
     if (P->fast_y) {
         MMX_TRY(fast_yi(P, st));
-        mark(P, st);
+        mark(P, st, PASS_YI);
     } else {
         const size_t b = lds_bytes_y(P, 1);
         allow_lds(yi_kernel, b);
         yi_kernel<<<dim3(ntx_y, g.n0, 3), dim3(kBlock), b, st>>>(g, P->fy, P->tiles.w_y, P->ybuf,
                                                                    P->xbuf);
         MMX_CHECK_LAUNCH();
-        mark(P, st);
+        mark(P, st, PASS_YI);
     }
     return MMX_OK;
 }
@@ -186,7 +193,7 @@ int demag_forward(DemagPlanImpl *P, const float *m, const float *msat_map, hipSt
     const size_t b = lds_bytes_x(P);
     allow_lds(xf_kernel<0>, b);
     const int nb = div_up(g.nrows, P->tiles.rows_x);
-    mark(P, st);
+    mark(P, st, PASS_START);
     if (P->fast_x) {
         MMX_TRY(fast_xf(P, 0, m, msat_map, nullptr, 0.0f, nullptr, nullptr, st));
     } else {
@@ -194,7 +201,7 @@ int demag_forward(DemagPlanImpl *P, const float *m, const float *msat_map, hipSt
                                                          nullptr, 0.0f, nullptr, nullptr, P->xbuf);
         MMX_CHECK_LAUNCH();
     }
-    mark(P, st);
+    mark(P, st, PASS_XF);
     return y_and_z_passes(P, st);
 }
 
@@ -206,7 +213,7 @@ int demag_forward_pred(DemagPlanImpl *P, const float *m0, const float *tau0, flo
     allow_lds(xf_kernel<1>, b);
     const int nb = div_up(g.nrows, P->tiles.rows_x);
     // the demag source is M = m * Msat_map, norm map = Msat map
-    mark(P, st);
+    mark(P, st, PASS_START);
     if (P->fast_x) {
         MMX_TRY(fast_xf(P, 1, m0, norm_map, tau0, h, norm_map, mpred, st));
     } else {
@@ -214,7 +221,7 @@ int demag_forward_pred(DemagPlanImpl *P, const float *m0, const float *tau0, flo
                                                          tau0, h, norm_map, mpred, P->xbuf);
         MMX_CHECK_LAUNCH();
     }
-    mark(P, st);
+    mark(P, st, PASS_XF);
     return y_and_z_passes(P, st);
 }
 
@@ -225,9 +232,10 @@ int demag_inverse(DemagPlanImpl *P, int mode, const FieldParams &fp, const Epilo
     const size_t b = lds_bytes_x(P);
     const int nb = div_up(g.nrows, P->tiles.rows_x);
     const int RB = P->tiles.rows_x;
+    const int pass_xi = (g.p0 == 1) ? 2 : 4;
     if (P->fast_x && fast_xi_applies(fp)) {
         MMX_TRY(fast_xi(P, mode, fp, io, st));
-        mark(P, st);
+        mark(P, st, pass_xi);
         return MMX_OK;
     }
     switch (mode) {
@@ -252,7 +260,7 @@ int demag_inverse(DemagPlanImpl *P, int mode, const FieldParams &fp, const Epilo
         return MMX_ERR_ARG;
     }
     MMX_CHECK_LAUNCH();
-    mark(P, st);
+    mark(P, st, pass_xi);
     return MMX_OK;
 }
 

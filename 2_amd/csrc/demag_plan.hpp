@@ -84,8 +84,13 @@ struct DemagPlanImpl {
     float *kern = nullptr;       // 6 x ko0 x ko1 x h2 (octant, real, 1/Np)
     float *real_kernel = nullptr;// optional: 6 x p0 p1 p2 real-space kernel
     bool fast_x = false, fast_y = false, fast_z = false; // register-FFT passes
-    hipEvent_t *events = nullptr; // when set: recorded before/after every pass
+    hipEvent_t *events = nullptr; // when set: recorded before/after every launch
+    int *event_pass = nullptr;    // pass id of the launch that each event ends
     int nevent = 0;
+    // ConvPlan: the real-space kernel comes from the caller (6 host float32
+    // arrays of the padded size, NULL = zero component), every component is
+    // transformed and the realness ratio is recorded, not enforced
+    const float *const *host_kernel = nullptr;
     // demagexch kernel build: real-space kernel Ms K_d + dex_cex K_ex
     bool dex = false;
     double dex_ms = 0, dex_cex = 0;
@@ -93,6 +98,8 @@ struct DemagPlanImpl {
 
 // Pass names in launch order (3-D grids; films run xf, y2d, xi).
 constexpr int kMaxPasses = 5;
+// Events recorded by one timed evaluation: the start and one per launch.
+constexpr int kMaxMarks = kMaxPasses + 1;
 
 // Kernel construction (kernel_build.hip)
 int build_demag_kernel(DemagPlanImpl *P, bool keep_real);
@@ -128,6 +135,18 @@ bool fast_xi_applies(const FieldParams &fp);
 
 } // namespace mmx
 
+namespace mmx {
+// Plan construction (plan.cpp); host_kernel: see DemagPlanImpl::host_kernel.
+int plan_create(mmx_demag_plan **plan, int n0, int n1, int n2, const double cell[3],
+                const int pbc[3], int kernel_type, int accuracy,
+                const float *const *host_kernel = nullptr);
+void plan_destroy(mmx_demag_plan *plan);
+} // namespace mmx
+
 struct mmx_demag_plan {
     mmx::DemagPlanImpl impl;
+};
+
+struct mmx_convplan {
+    mmx_demag_plan *plan = nullptr;
 };

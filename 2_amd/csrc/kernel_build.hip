@@ -397,6 +397,32 @@ __global__ void __launch_bounds__(kBlock) extract_octant(const double2 *__restri
     block_max_to(good, maxgood);
 }
 
+// The octant storage takes K at kz > p0/2 or ky > p1/2 from its mirror
+// point, sign-flipped for the components odd in that axis (XZ and XY in axis
+// 0, YZ and XY in axis 1).  For a kernel given by the caller this maximum
+// deviation of the full FP64 spectrum from that rule is checked.
+__global__ void __launch_bounds__(kBlock) octant_mirror_dev(const double2 *__restrict__ full,
+                                                            int p0, int p1, int p2, int h2,
+                                                            int odd0, int odd1,
+                                                            unsigned int *maxdev)
+{
+    const long n = (long)p0 * p1 * h2;
+    float dev = 0.0f;
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
+        const int kx = (int)(t % h2);
+        const int ky = (int)((t / h2) % p1);
+        const int kz = (int)(t / ((long)h2 * p1));
+        const bool mz = kz > p0 / 2, my = ky > p1 / 2;
+        if (!mz && !my) continue;
+        const int kzp = mz ? p0 - kz : kz, kyp = my ? p1 - ky : ky;
+        const double sign = ((mz && odd0) ? -1.0 : 1.0) * ((my && odd1) ? -1.0 : 1.0);
+        const double a = full[((long)kz * p1 + ky) * p2 + kx].x;
+        const double b = full[((long)kzp * p1 + kyp) * p2 + kx].x;
+        dev = fmaxf(dev, (float)fabs(a - sign * b));
+    }
+    block_max_to(dev, maxdev);
+}
+
 // ------------------------------------------------------------ demagexch
 //
 // Exch6NgbrKernel (modules/kernel_exchange.go:102-124): the 6-neighbour
@@ -531,6 +557,17 @@ int build_demag_kernel(DemagPlanImpl *P, bool keep_real)
             newell_octant<<<dim3(div_up(ne, kBlock)), dim3(kBlock)>>>(a, oct);
             maxabs_oct6<<<dim3(gb), dim3(kBlock)>>>(oct, ne, stats);
             if (hipGetLastError() != hipSuccess) { set_error("newell_octant launch failed"); fail(MMX_ERR_HIP); break; }
+        } else if (P->host_kernel) {
+            // ConvPlan.loadKernel (convplan.go:85-115): the caller's kernel
+            for (int c = 0; c < 6; c++) {
+                if (!P->host_kernel[c]) continue;
+                if (hipMemcpy(K6 + c * Pn, P->host_kernel[c], sizeof(float) * Pn,
+                              hipMemcpyHostToDevice) != hipSuccess) { fail(MMX_ERR_HIP); break; }
+            }
+            if (rc) break;
+            for (int c = 0; c < 6; c++)
+                maxabs_kernel<<<dim3(gb), dim3(kBlock)>>>(K6 + c * Pn, Pn, stats + c);
+            if (hipGetLastError() != hipSuccess) { set_error("maxabs launch failed"); fail(MMX_ERR_HIP); break; }
         } else {
             // ranges of Kernel_Arne (kernel_magnetostatic.go:50-65)
             ArneArgs a{};
@@ -605,7 +642,9 @@ int build_demag_kernel(DemagPlanImpl *P, bool keep_real)
             float *dst = P->kern + c;
             // (ConvPlan.loadKernel, used by demagexch, transforms every
             // component: convplan.go:85-115)
-            if (!P->dex && (kmax == 0.0f || cmax[c] / kmax < 1e-5f)) {
+            const bool unused = P->host_kernel ? P->host_kernel[c] == nullptr
+                                               : (!P->dex && (kmax == 0.0f || cmax[c] / kmax < 1e-5f));
+            if (unused) {
                 zero_strided<<<dim3(1024), dim3(kBlock)>>>(dst, octant);
                 if (hipGetLastError() != hipSuccess) { set_error("kernel zero launch failed"); fail(MMX_ERR_HIP); break; }
                 continue;
@@ -625,6 +664,12 @@ int build_demag_kernel(DemagPlanImpl *P, bool keep_real)
             }
             extract_octant<<<dim3(1024), dim3(kBlock)>>>(full, p[0], p[1], p[2], g.ko0, g.ko1, g.h2,
                                                          inv, dst, stats + 8, stats + 9);
+            if (P->host_kernel) {
+                // components XX,YY,ZZ,YZ,XZ,XY: odd in axis 0 for XZ, XY; in axis 1 for YZ, XY
+                const int odd0 = (c == 4 || c == 5), odd1 = (c == 3 || c == 5);
+                octant_mirror_dev<<<dim3(1024), dim3(kBlock)>>>(full, p[0], p[1], p[2], g.h2, odd0,
+                                                                odd1, stats + 10);
+            }
             if (hipGetLastError() != hipSuccess) { set_error("kernel FFT launch failed"); fail(MMX_ERR_HIP); break; }
         }
         if (rc) break;
@@ -633,7 +678,19 @@ int build_demag_kernel(DemagPlanImpl *P, bool keep_real)
         std::memcpy(&bad, &hs[8], 4);
         std::memcpy(&good, &hs[9], 4);
         P->bad_over_good = good > 0 ? (double)bad / (double)good : 0.0;
-        if (P->bad_over_good > 1e-5) {
+        if (P->host_kernel) {
+            // ConvPlan only warns about imaginary parts (convplan.go:143-152);
+            // the octant storage needs the mirror symmetry, so that is enforced
+            float mdev;
+            std::memcpy(&mdev, &hs[10], 4);
+            if (good > 0 && (double)mdev / (double)good > 1e-5) {
+                set_error("convolution kernel: transformed kernel deviates from the mirror "
+                          "symmetry of a demag/exchange kernel by %g of its maximum",
+                          (double)mdev / (double)good);
+                fail(MMX_ERR_ARG);
+                break;
+            }
+        } else if (P->bad_over_good > 1e-5) {
             set_error("FFT kernel max bad/good part = %g", P->bad_over_good);
             fail(MMX_ERR_STATE);
             break;

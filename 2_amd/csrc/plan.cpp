@@ -161,12 +161,15 @@ static void plan_free(DemagPlanImpl *P)
 }
 
 int plan_create(mmx_demag_plan **plan, int n0, int n1, int n2, const double cell[3],
-                const int pbc[3], int kernel_type, int accuracy)
+                const int pbc[3], int kernel_type, int accuracy,
+                const float *const *host_kernel)
 {
     MMX_REQUIRE(plan, MMX_ERR_ARG, "demag plan: NULL output pointer");
     const bool keep = (kernel_type & MMX_KEEP_REAL_KERNEL) != 0;
     mmx_demag_plan *pl = new mmx_demag_plan();
+    pl->impl.host_kernel = host_kernel;
     int rc = plan_init(&pl->impl, n0, n1, n2, cell, pbc, kernel_type & 0xff, accuracy, keep);
+    pl->impl.host_kernel = nullptr; // the caller's arrays are not kept
     if (rc != MMX_OK) {
         plan_free(&pl->impl);
         delete pl;
@@ -208,7 +211,7 @@ int mmx_demag_plan_create(mmx_demag_plan **plan, int n0, int n1, int n2, double 
 {
     const double cell[3] = {cell0, cell1, cell2};
     const int pbc[3] = {pbc0, pbc1, pbc2};
-    return plan_create(plan, n0, n1, n2, cell, pbc, kernel_type, accuracy);
+    return plan_create(plan, n0, n1, n2, cell, pbc, kernel_type, accuracy, nullptr);
 }
 
 int mmx_demag_plan_destroy(mmx_demag_plan *plan)
@@ -268,6 +271,62 @@ int mmx_demag_plan_real_kernel(const mmx_demag_plan *plan, float *host)
     const DemagGeom &g = plan->impl.g;
     MMX_HIP(hipMemcpy(host, plan->impl.real_kernel,
                       sizeof(float) * 6 * (size_t)g.p0 * g.p1 * g.p2, hipMemcpyDeviceToHost));
+    return MMX_OK;
+}
+
+int mmx_convplan_create(mmx_convplan **cp, const int dataSize[3], const int logicSize[3],
+                        const float *const kernel[6])
+{
+    MMX_REQUIRE(cp && dataSize && logicSize && kernel, MMX_ERR_ARG,
+                "mmx_convplan_create: NULL argument");
+    int pbc[3];
+    for (int a = 0; a < 3; a++) {
+        const int n = dataSize[a], l = logicSize[a];
+        MMX_REQUIRE(n > 0 && (l == n || (n > 1 && l == 2 * n)), MMX_ERR_SIZE,
+                    "mmx_convplan_create: axis %d: kernel size %d for data size %d (must be the "
+                    "data size or twice it)", a, l, n);
+        pbc[a] = (l == n && n > 1) ? 1 : 0;
+    }
+    bool any = false;
+    for (int c = 0; c < 6; c++) any = any || kernel[c] != nullptr;
+    MMX_REQUIRE(any, MMX_ERR_ARG, "mmx_convplan_create: every kernel component is NULL");
+    // the cell size only matters for a built kernel; a unit cell is passed
+    const double cell[3] = {1.0, 1.0, 1.0};
+    mmx_demag_plan *pl = nullptr;
+    MMX_TRY(plan_create(&pl, dataSize[0], dataSize[1], dataSize[2], cell, pbc, MMX_KERNEL_ARNE, 6,
+                        kernel));
+    mmx_convplan *c = new mmx_convplan();
+    c->plan = pl;
+    *cp = c;
+    return MMX_OK;
+}
+
+int mmx_convplan_destroy(mmx_convplan *cp)
+{
+    if (!cp) return MMX_OK;
+    plan_destroy(cp->plan);
+    delete cp;
+    return MMX_OK;
+}
+
+int mmx_convplan_convolve(mmx_convplan *cp, const float *in, float *out, void *stream)
+{
+    MMX_REQUIRE(cp && cp->plan && in && out, MMX_ERR_ARG, "mmx_convplan_convolve: NULL argument");
+    MMX_REQUIRE(in != out, MMX_ERR_ARG, "mmx_convplan_convolve: in and out must differ");
+    DemagPlanImpl *P = &cp->plan->impl;
+    const hipStream_t st = as_stream(stream);
+    MMX_TRY(demag_forward(P, in, nullptr, st));
+    FieldParams fp{};
+    fp.demag_scale = 1.0f;
+    EpilogueIO io{};
+    io.hout = out;
+    return demag_inverse(P, EPI_B, fp, io, st);
+}
+
+int mmx_convplan_info(const mmx_convplan *cp, double *bad_over_good)
+{
+    MMX_REQUIRE(cp && cp->plan && bad_over_good, MMX_ERR_ARG, "mmx_convplan_info: NULL argument");
+    *bad_over_good = cp->plan->impl.bad_over_good;
     return MMX_OK;
 }
 

@@ -37,9 +37,6 @@
 
 namespace mmx {
 
-int plan_create(mmx_demag_plan **plan, int n0, int n1, int n2, const double cell[3],
-                const int pbc[3], int kernel_type, int accuracy);
-void plan_destroy(mmx_demag_plan *plan);
 
 typedef unsigned short bf16_t;
 typedef short sh4 __attribute__((ext_vector_type(4)));

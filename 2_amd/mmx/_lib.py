@@ -83,7 +83,45 @@ _SIGS = {
     "mmx_batch_tune": (_i, [_vp, _i, C.POINTER(_i)]),
     "mmx_batch_set_lds": (_i, [_vp, C.POINTER(_i)]),
     "mmx_batch_info": (_i, [_vp, C.POINTER(_i), C.POINTER(_i)]),
+    "mmx_convplan_create": (_i, [C.POINTER(_vp), _vp, _vp, _vp]),
+    "mmx_convplan_destroy": (_i, [_vp]),
+    "mmx_convplan_convolve": (_i, [_vp, _fp, _fp, _vp]),
+    "mmx_convplan_info": (_i, [_vp, C.POINTER(_d)]),
 }
+
+# include/mmx_libmumax2.h: libmumax2's signatures (float** = address of an
+# array of device pointers, hipStream_t* = address of an array of streams);
+# no return value (errors abort the process)
+_pp = C.c_void_p
+_LM2 = {
+    "mmx_setUsedGPUs": [_vp, _i],
+    "mmx_normalizeAsync": [_pp] * 4 + [_vp, _i],
+    "mmx_torqueAsync": [_pp] * 10 + [_f, _vp, _i],
+    "mmx_exchange6Async": [_pp] * 8 + [_f, _i, _i, _i, _i, _i, _i, _f, _f, _f, _vp],
+    "mmx_uniaxialAnisotropyAsync": [_pp] * 8 + [_f, _pp, _f, _pp, _f, _pp, _f, _vp, _i],
+    "mmx_zhangli_async": [_pp] * 13 + [_f] * 6 + [_i] * 3 + [_f] * 3 + [_i] * 3 + [_vp],
+    "mmx_maddAsync": [_pp, _pp, _pp, _f, _vp, _i],
+    "mmx_madd1Async": [_pp, _pp, _f, _vp, _i],
+    "mmx_madd2Async": [_pp, _pp, _f, _pp, _f, _vp, _i],
+    "mmx_vecMaddAsync": [_pp] * 9 + [_f, _f, _f, _vp, _i],
+    "mmx_mulAsync": [_pp, _pp, _pp, _vp, _i],
+    "mmx_cmaddAsync": [_pp, _f, _f, _pp, _pp, _vp, _i],
+    "mmx_kernelMulMicromag3DAsync": [_pp] * 9 + [_vp, _i],
+    "mmx_copyPad3DAsync": [_pp, _i, _i, _i, _pp, _i, _i, _i, _i, _vp],
+    "mmx_copyUnPad3DAsync": [_pp, _i, _i, _i, _pp, _i, _i, _i, _i, _vp],
+    "mmx_zeroArrayAsync": [_pp, _i, _vp],
+    "mmx_partialSumAsync": [_pp, _pp, _i, _i, _i, _vp],
+    "mmx_partialSDotAsync": [_pp, _pp, _pp, _i, _i, _i, _vp],
+    "mmx_partialMaxAsync": [_pp, _pp, _i, _i, _i, _vp],
+    "mmx_partialMinAsync": [_pp, _pp, _i, _i, _i, _vp],
+    "mmx_partialMaxAbsAsync": [_pp, _pp, _i, _i, _i, _vp],
+    "mmx_partialMaxDiffAsync": [_pp, _pp, _pp, _i, _i, _i, _vp],
+    "mmx_partialMaxSumAsync": [_pp, _pp, _pp, _i, _i, _i, _vp],
+    "mmx_partialMaxNorm3SqAsync": [_pp] * 4 + [_i, _i, _i, _vp],
+    "mmx_partialMaxNorm3SqDiffAsync": [_pp] * 7 + [_i, _i, _i, _vp],
+}
+for _name, _args in _LM2.items():
+    _SIGS[_name] = (None, _args)
 
 EXPORTED_SYMBOLS = tuple(sorted(_SIGS))
 

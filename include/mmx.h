@@ -220,6 +220,36 @@ int mmx_demag_plan_kernel(const mmx_demag_plan *plan, float *host);
 int mmx_demag_plan_real_kernel(const mmx_demag_plan *plan, float *host);
 
 /* ------------------------------------------------------------------------
+ * Convolution plan: ConvPlan (src/mumax/gpu/convplan.go:22-231), the
+ * convolution that the demagexch module runs with its combined kernel
+ * (src/mumax/modules/mod_demagexchange.go:26-205).
+ * ---------------------------------------------------------------------- */
+typedef struct mmx_convplan mmx_convplan;
+
+/* ConvPlan.Init + loadKernel (convplan.go:33-115).  dataSize: the non-zero
+ * input block (internal order); logicSize: the kernel's size, per axis the
+ * data size (periodic axis, or length 1) or twice it (zero padding), as
+ * PadSize gives it (engine/zeropad.go:12-22); kernel: 6 host float32 arrays of
+ * logicSize (components XX,YY,ZZ,YZ,XZ,XY, common/index.go:57-63; NULL =
+ * unused), not normalised (the plan applies 1/prod(logicSize)).  The kernel is
+ * transformed in FP64 and stored as its octant (6 real values per point), which
+ * needs the mirror symmetry of a demag/exchange kernel: even in every axis
+ * except, for an off-diagonal component, its two axes; a kernel without it is
+ * refused (MMX_ERR_ARG).  The imaginary-part ratio is recorded
+ * (mmx_convplan_info), as ConvPlan warns about it (convplan.go:143-152).
+ * Allocates the plan's device buffers. */
+int mmx_convplan_create(mmx_convplan **cp, const int dataSize[3],
+                        const int logicSize[3], const float *const kernel[6]);
+int mmx_convplan_destroy(mmx_convplan *cp);
+/* ConvPlan.Convolve (convplan.go:163-176): out = kernel (*) in for the 3
+ * components (component c at base + c*N, N = cells of dataSize; in != out),
+ * asynchronous on `stream`. */
+int mmx_convplan_convolve(mmx_convplan *cp, const float *in, float *out,
+                          void *stream);
+/* max |Im| / max |Re| of the transformed kernel */
+int mmx_convplan_info(const mmx_convplan *cp, double *bad_over_good);
+
+/* ------------------------------------------------------------------------
  * Simulation engine for this path: the Engine/Quant/module/solver pieces of
  * src/mumax/engine and src/mumax/modules that drive the kernels above
  * (micromagnetism = demag + exchange6 + zeeman + llg + solver/rk12 +

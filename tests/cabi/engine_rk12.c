@@ -1,11 +1,15 @@
 /* engine_rk12.c -- the Go engine's field and solver sequence written against
- * the C ABI alone, as the cgo wrappers of mumax/gpu would drive it: the
- * caller allocates every device buffer (array.go:182-192), passes component
- * pointers and the stream, NULL for uniform maps, and checks each status.
+ * the C ABI alone, as the cgo wrappers of mumax/gpu drive libmumax2: every
+ * array is a float** of per-device pointers (one GPU: one element), streams a
+ * hipStream_t*, the argument lists of src/libmumax2's headers
+ * (include/mmx_libmumax2.h); the caller allocates every device buffer
+ * (array.go:182-192), NULL is a uniform map, and the Reductor's partials are
+ * finished on the host (reductor.go:19-212).
  *
  *   H_eff = B/mu0 + H_ex + B_ext/mu0   (SumUpdater, update_sum.go:29-97;
  *                                        B from MaxwellPlan.update,
- *                                        maxwell_convolution.go:272-309)
+ *                                        maxwell_convolution.go:272-309,
+ *                                        replaced by mmx_demag_update_b)
  *   tau0  = torque(m, H_eff(m))        (mod_llg.go:59-70)
  *   m1    = normalize(m + h tau0)      (solve_rk12.go:97-123)
  *   tau1  = torque(m1, H_eff(m1))
@@ -25,8 +29,12 @@
 #include <hip/hip_runtime_api.h>
 
 #include "mmx.h"
+#include "mmx_libmumax2.h"
 
 #define MU0 (4.0 * 3.14159265358979323846 * 1e-7)
+
+/* a one-device float** (the Go Array's pointer slice) */
+#define DEV(p) ((float *[1]){(float *)(p)})
 
 static void die(const char *what, int rc)
 {
@@ -62,9 +70,11 @@ struct engine {
     float cellf[3];
     double msat, aex, alpha, bext[3];
     mmx_demag_plan *plan;
-    mmx_reductor *red;
-    hipStream_t st;
+    hipStream_t st[1];
     float *b, *hex, *h;
+    float *part;      /* the Reductor's device partials */
+    float *hpart;
+    int blocks;
 };
 
 /* H_eff at m into e->h */
@@ -72,24 +82,40 @@ static void heff(struct engine *e, const float *m)
 {
     const long N = e->N;
     CHECK(mmx_demag_update_b(e->plan, e->b, e->b + N, e->b + 2 * N, m, m + N, m + 2 * N, NULL,
-                             (float)e->msat, e->st));
-    CHECK(mmx_exchange6(e->hex, e->hex + N, e->hex + 2 * N, m, m + N, m + 2 * N, NULL, NULL,
-                        (float)((2 * e->aex) / (MU0 * e->msat)), e->n[0], e->n[1], e->n[2], 0, 0, 0,
-                        e->cellf[0], e->cellf[1], e->cellf[2], e->st));
+                             (float)e->msat, e->st[0]));
+    mmx_exchange6Async(DEV(e->hex), DEV(e->hex + N), DEV(e->hex + 2 * N), DEV(m), DEV(m + N),
+                       DEV(m + 2 * N), NULL, NULL, (float)((2 * e->aex) / (MU0 * e->msat)),
+                       e->n[0], e->n[1], e->n[2], 0, 0, 0, e->cellf[0], e->cellf[1], e->cellf[2],
+                       e->st);
     const float inv_mu0 = (float)(1.0 / MU0);
-    CHECK(mmx_vec_madd(e->h, e->h + N, e->h + 2 * N, e->hex, e->hex + N, e->hex + 2 * N, e->b,
-                       e->b + N, e->b + 2 * N, inv_mu0, inv_mu0, inv_mu0, (int)N, e->st));
-    CHECK(mmx_vec_madd(e->h, e->h + N, e->h + 2 * N, e->h, e->h + N, e->h + 2 * N, NULL, NULL,
-                       NULL, (float)(e->bext[0] / MU0), (float)(e->bext[1] / MU0),
-                       (float)(e->bext[2] / MU0), (int)N, e->st));
+    mmx_vecMaddAsync(DEV(e->h), DEV(e->h + N), DEV(e->h + 2 * N), DEV(e->hex), DEV(e->hex + N),
+                     DEV(e->hex + 2 * N), DEV(e->b), DEV(e->b + N), DEV(e->b + 2 * N), inv_mu0,
+                     inv_mu0, inv_mu0, e->st, (int)N);
+    mmx_vecMaddAsync(DEV(e->h), DEV(e->h + N), DEV(e->h + 2 * N), DEV(e->h), DEV(e->h + N),
+                     DEV(e->h + 2 * N), NULL, NULL, NULL, (float)(e->bext[0] / MU0),
+                     (float)(e->bext[1] / MU0), (float)(e->bext[2] / MU0), e->st, (int)N);
 }
 
 static void torque(struct engine *e, float *tau, const float *m)
 {
     const long N = e->N;
     heff(e, m);
-    CHECK(mmx_torque(tau, tau + N, tau + 2 * N, m, m + N, m + 2 * N, e->h, e->h + N, e->h + 2 * N,
-                     NULL, (float)e->alpha, (int)N, e->st));
+    mmx_torqueAsync(DEV(tau), DEV(tau + N), DEV(tau + 2 * N), DEV(m), DEV(m + N), DEV(m + 2 * N),
+                    DEV(e->h), DEV(e->h + N), DEV(e->h + 2 * N), NULL, (float)e->alpha, e->st,
+                    (int)N);
+}
+
+/* Reductor.MaxDiff: block partials on the device, the maximum on the host */
+static float max_diff(struct engine *e, const float *a, const float *b, long n)
+{
+    mmx_partialMaxDiffAsync(DEV(a), DEV(b), DEV(e->part), e->blocks, 512, (int)n, e->st);
+    HCHECK(hipMemcpyAsync(e->hpart, e->part, sizeof(float) * e->blocks, hipMemcpyDeviceToHost,
+                          e->st[0]));
+    HCHECK(hipStreamSynchronize(e->st[0]));
+    float r = e->hpart[0];
+    for (int i = 1; i < e->blocks; i++)
+        if (e->hpart[i] > r) r = e->hpart[i];
+    return r;
 }
 
 int main(int argc, char **argv)
@@ -126,11 +152,15 @@ int main(int argc, char **argv)
     fclose(f);
     for (int d = 0; d < 3; d++) e.cellf[d] = (float)cell[d];
 
-    CHECK(mmx_init(0));
-    HCHECK(hipStreamCreateWithFlags(&e.st, hipStreamNonBlocking));
+    int gpus[1] = {0};
+    mmx_setUsedGPUs(gpus, 1);
+    HCHECK(hipStreamCreateWithFlags(&e.st[0], hipStreamNonBlocking));
     CHECK(mmx_demag_plan_create(&e.plan, e.n[0], e.n[1], e.n[2], cell[0], cell[1], cell[2], 0, 0, 0,
                                 MMX_KERNEL_ARNE, 6));
-    CHECK(mmx_reductor_create(&e.red, 3 * N));
+    e.blocks = (int)((3 * N + 1023) / 1024);
+    if (e.blocks > 2048) e.blocks = 2048;
+    HCHECK(hipMalloc((void **)&e.part, sizeof(float) * e.blocks));
+    e.hpart = (float *)malloc(sizeof(float) * e.blocks);
     float *m = dev3(N), *m1 = dev3(N), *tau0 = dev3(N), *tau1 = dev3(N);
     e.b = dev3(N);
     e.hex = dev3(N);
@@ -148,25 +178,24 @@ int main(int argc, char **argv)
     double gamma = 0;
     CHECK(mmx_sim_getv(sim, "gamma", &gamma, 1));
 
-    CHECK(mmx_normalize(m, m + N, m + 2 * N, NULL, (int)N, e.st));
+    mmx_normalizeAsync(DEV(m), DEV(m + N), DEV(m + 2 * N), NULL, e.st, (int)N);
     for (int k = 0; k < ntries; k++) {
         const double dtk = dt * (1 + 0.1 * k);
         const float h = (float)(dtk * gamma);
         torque(&e, tau0, m);
-        CHECK(mmx_madd(m1, m, tau0, h, (int)(3 * N), e.st));
-        CHECK(mmx_normalize(m1, m1 + N, m1 + 2 * N, NULL, (int)N, e.st));
+        mmx_maddAsync(DEV(m1), DEV(m), DEV(tau0), h, e.st, (int)(3 * N));
+        mmx_normalizeAsync(DEV(m1), DEV(m1 + N), DEV(m1 + 2 * N), NULL, e.st, (int)N);
         torque(&e, tau1, m1);
-        float maxdiff = 0;
-        CHECK(mmx_reductor_maxdiff(e.red, tau1, tau0, 3 * N, e.st, &maxdiff));
+        const float maxdiff = max_diff(&e, tau1, tau0, 3 * N);
         errs[2 * k] = (double)(maxdiff * h);
-        CHECK(mmx_madd2(m1, tau1, 0.5f * h, tau0, -0.5f * h, (int)(3 * N), e.st));
-        CHECK(mmx_normalize(m1, m1 + N, m1 + 2 * N, NULL, (int)N, e.st));
+        mmx_madd2Async(DEV(m1), DEV(tau1), 0.5f * h, DEV(tau0), -0.5f * h, e.st, (int)(3 * N));
+        mmx_normalizeAsync(DEV(m1), DEV(m1 + N), DEV(m1 + 2 * N), NULL, e.st, (int)N);
         float *t = m;
         m = m1;
         m1 = t;
         CHECK(mmx_sim_try_step(sim, dtk, &errs[2 * k + 1]));
     }
-    HCHECK(hipStreamSynchronize(e.st));
+    HCHECK(hipStreamSynchronize(e.st[0]));
 
     FILE *g = fopen(argv[2], "wb");
     if (!g) {
@@ -181,7 +210,8 @@ int main(int argc, char **argv)
     fclose(g);
 
     CHECK(mmx_sim_destroy(sim));
-    CHECK(mmx_reductor_destroy(e.red));
+    HCHECK(hipFree(e.part));
+    free(e.hpart);
     CHECK(mmx_demag_plan_destroy(e.plan));
     HCHECK(hipFree(m));
     HCHECK(hipFree(m1));
@@ -190,7 +220,7 @@ int main(int argc, char **argv)
     HCHECK(hipFree(e.b));
     HCHECK(hipFree(e.hex));
     HCHECK(hipFree(e.h));
-    HCHECK(hipStreamDestroy(e.st));
+    HCHECK(hipStreamDestroy(e.st[0]));
     printf("engine_rk12: %d tries on %d x %d x %d done\n", ntries, e.n[0], e.n[1], e.n[2]);
     return 0;
 }

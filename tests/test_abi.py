@@ -1,6 +1,7 @@
 """The C-ABI library builds, loads without a GPU and exports every entry point
 that include/mmx.h declares (no compute calls here)."""
 import ctypes
+import glob
 import os
 import re
 import subprocess
@@ -8,14 +9,16 @@ import subprocess
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-HEADER = os.path.join(ROOT, "include", "mmx.h")
+HEADERS = sorted(glob.glob(os.path.join(ROOT, "include", "*.h")))
 LIB = os.path.join(ROOT, "2_amd", "mmx", "libmmx.so")
 
 
 def _declared():
-    text = open(HEADER).read()
-    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"\b(mmx_[a-z0-9_]+)\s*\(", text)))
+    names = set()
+    for h in HEADERS:
+        text = re.sub(r"/\*.*?\*/", "", open(h).read(), flags=re.S)
+        names.update(re.findall(r"\b(mmx_[A-Za-z0-9_]+)\s*\(", text))
+    return sorted(names)
 
 
 @pytest.fixture(scope="module")
@@ -30,14 +33,17 @@ def test_header_declares_entry_points():
     assert len(names) >= 40
     for must in ("mmx_exchange6", "mmx_torque", "mmx_normalize", "mmx_zhangli",
                  "mmx_uniaxial_anisotropy", "mmx_demag_update_b", "mmx_sim_step",
-                 "mmx_reductor_maxdiff", "mmx_madd2", "mmx_vec_madd"):
+                 "mmx_reductor_maxdiff", "mmx_madd2", "mmx_vec_madd",
+                 # libmumax2's own signatures (include/mmx_libmumax2.h)
+                 "mmx_torqueAsync", "mmx_exchange6Async", "mmx_kernelMulMicromag3DAsync",
+                 "mmx_partialMaxDiffAsync", "mmx_setUsedGPUs", "mmx_convplan_convolve"):
         assert must in names
 
 
 def test_library_exports_every_declared_symbol(built):
     out = subprocess.run(["nm", "-D", "--defined-only", built], capture_output=True, text=True,
                          check=True).stdout
-    exported = set(re.findall(r"\s[TW]\s+(mmx_[a-z0-9_]+)$", out, flags=re.M))
+    exported = set(re.findall(r"\s[TW]\s+(mmx_[A-Za-z0-9_]+)$", out, flags=re.M))
     missing = [n for n in _declared() if n not in exported]
     assert not missing, missing
 
@@ -65,3 +71,13 @@ def test_product_does_not_touch_oracle():
                     for pat in ("import oracle", "from oracle", "libmmx_oracle", "orc_",
                                 "mmx_oracle"):
                         assert pat not in txt, (os.path.join(dirpath, f), pat)
+
+
+def test_libmumax2_entry_points_abort_on_bad_arguments(built):
+    """The libmumax2-shaped calls return nothing and abort on an error, as the
+    reference's gpu_safe does (gpu_safe.h:36-41); checked before any HIP call."""
+    code = ("import ctypes as C; L = C.CDLL(%r); "
+            "L.mmx_setUsedGPUs((C.c_int * 2)(0, 1), 2)" % built)
+    r = subprocess.run(["python", "-c", code], capture_output=True, text=True)
+    assert r.returncode == -6, (r.returncode, r.stderr)
+    assert "setUsedGPUs" in r.stderr and "one GPU" in r.stderr

@@ -1,12 +1,14 @@
 """The C ABI driven from compiled host code, as the cgo wrappers would.
 
 tests/cabi/engine_rk12.c (plain C, built by __graft_entry__.build() against
-include/mmx.h and libmmx.so) allocates its own device buffers and composes the
-Go engine's H_eff (demag B, exchange6, Zeeman, SumUpdater) and rk12 tries from
-the per-kernel entry points (normalize, torque, exchange6, vec_madd, madd,
-madd2, the reductor's MaxDiff, the demag plan), and runs the same tries
-through the fused simulation.  Both are replayed by the FP64 oracle: m to
-1e-5, the error estimates to 1e-3 relative.
+include/mmx.h, include/mmx_libmumax2.h and libmmx.so) allocates its own device
+buffers and composes the Go engine's H_eff (demag B, exchange6, Zeeman,
+SumUpdater) and rk12 tries from the libmumax2-shaped entry points (float**
+arrays, hipStream_t* streams: normalizeAsync, torqueAsync, exchange6Async,
+vecMaddAsync, maddAsync, madd2Async, partialMaxDiffAsync finished on the
+host) and the demag plan, and runs the same tries through the fused
+simulation.  Both are replayed by the FP64 oracle: m to 1e-5, the error
+estimates to 1e-3 relative.
 """
 import os
 import struct
