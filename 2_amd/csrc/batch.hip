@@ -59,18 +59,22 @@ struct BatchState {
 };
 
 // ------------------------------------------------------------- kernels
+//
+// Each pass is a device function of one virtual block `vb` (a tile of one
+// simulation, from the pass's block table), called by the per-pass kernels
+// with vb = blockIdx.x and by the persistent kernel for vb = blockIdx.x,
+// blockIdx.x + gridDim.x, ...
 
 template <int STAGE>
-__global__ void __launch_bounds__(kBlock)
-bxf_kernel(const BatchSim *__restrict__ sims, const BatchState *__restrict__ st,
-           const int *__restrict__ map)
+__device__ __forceinline__ void bxf_block(const BatchSim *__restrict__ sims,
+                                          const BatchState *__restrict__ st,
+                                          const int *__restrict__ map, int vb, float2 *sm)
 {
-    extern __shared__ float2 sm[];
-    const int s = map[blockIdx.x];
+    const int s = map[vb];
     const BatchSim &S = sims[s];
     const BatchState &Z = st[s];
     if (!Z.active || (STAGE == 0 && !Z.need_stage0)) return;
-    const long row0 = (long)(blockIdx.x - S.first[BP_XF]) * S.tile[BP_XF];
+    const long row0 = (long)(vb - S.first[BP_XF]) * S.tile[BP_XF];
     const float *m = S.mbuf[Z.cur];
     if (STAGE == 0)
         xf_tile<0>(S.g, S.fx, S.tile[BP_XF], row0, m, S.msat_map, nullptr, 0.0f, nullptr, nullptr,
@@ -81,16 +85,15 @@ bxf_kernel(const BatchSim *__restrict__ sims, const BatchState *__restrict__ st,
 }
 
 template <int STAGE>
-__global__ void __launch_bounds__(kBlock)
-by_kernel(const BatchSim *__restrict__ sims, const BatchState *__restrict__ st,
-          const int *__restrict__ map)
+__device__ __forceinline__ void by_block(const BatchSim *__restrict__ sims,
+                                         const BatchState *__restrict__ st,
+                                         const int *__restrict__ map, int vb, float2 *sm)
 {
-    extern __shared__ float2 sm[];
-    const int s = map[blockIdx.x];
+    const int s = map[vb];
     const BatchSim &S = sims[s];
     const BatchState &Z = st[s];
     if (!Z.active || (STAGE == 0 && !Z.need_stage0)) return;
-    const int b = blockIdx.x - S.first[BP_Y], W = S.tile[BP_Y], ntx = S.ntx[BP_Y];
+    const int b = vb - S.first[BP_Y], W = S.tile[BP_Y], ntx = S.ntx[BP_Y];
     if (S.g.p0 == 1) {
         y2d_tile(S.g, S.fy, W, b * W, S.kern, S.xbuf, sm);
     } else {
@@ -100,16 +103,15 @@ by_kernel(const BatchSim *__restrict__ sims, const BatchState *__restrict__ st,
 }
 
 template <int STAGE>
-__global__ void __launch_bounds__(kBlock)
-bz_kernel(const BatchSim *__restrict__ sims, const BatchState *__restrict__ st,
-          const int *__restrict__ map)
+__device__ __forceinline__ void bz_block(const BatchSim *__restrict__ sims,
+                                         const BatchState *__restrict__ st,
+                                         const int *__restrict__ map, int vb, float2 *sm)
 {
-    extern __shared__ float2 sm[];
-    const int s = map[blockIdx.x];
+    const int s = map[vb];
     const BatchSim &S = sims[s];
     const BatchState &Z = st[s];
     if (!Z.active || (STAGE == 0 && !Z.need_stage0)) return;
-    const int b = blockIdx.x - S.first[BP_Z], W = S.tile[BP_Z], ntx = S.ntx[BP_Z];
+    const int b = vb - S.first[BP_Z], W = S.tile[BP_Z], ntx = S.ntx[BP_Z];
     if (S.zsmall) {
         const long col = (long)b * kBlock + threadIdx.x;
         if (col >= (long)S.g.p1 * S.g.h2) return;
@@ -121,16 +123,15 @@ bz_kernel(const BatchSim *__restrict__ sims, const BatchState *__restrict__ st,
 }
 
 template <int STAGE>
-__global__ void __launch_bounds__(kBlock)
-byi_kernel(const BatchSim *__restrict__ sims, const BatchState *__restrict__ st,
-           const int *__restrict__ map)
+__device__ __forceinline__ void byi_block(const BatchSim *__restrict__ sims,
+                                          const BatchState *__restrict__ st,
+                                          const int *__restrict__ map, int vb, float2 *sm)
 {
-    extern __shared__ float2 sm[];
-    const int s = map[blockIdx.x];
+    const int s = map[vb];
     const BatchSim &S = sims[s];
     const BatchState &Z = st[s];
     if (!Z.active || (STAGE == 0 && !Z.need_stage0)) return;
-    const int b = blockIdx.x - S.first[BP_YI], W = S.tile[BP_YI], ntx = S.ntx[BP_YI];
+    const int b = vb - S.first[BP_YI], W = S.tile[BP_YI], ntx = S.ntx[BP_YI];
     const int kt = b % ntx, i = (b / ntx) % S.g.n0, c = b / (ntx * S.g.n0);
     yi_tile(S.g, S.fy, W, kt * W, i, c, S.ybuf, S.xbuf, sm);
 }
@@ -138,16 +139,15 @@ byi_kernel(const BatchSim *__restrict__ sims, const BatchState *__restrict__ st,
 // STAGE 0: tau0 = torque(m) (solve_rk12.go:97-115); STAGE 1: corrector,
 // normalised m into the other buffer, max |tau1 - tau0| into err_bits.
 template <int STAGE>
-__global__ void __launch_bounds__(kBlock)
-bxi_kernel(const BatchSim *__restrict__ sims, BatchState *__restrict__ st,
-           const int *__restrict__ map)
+__device__ __forceinline__ void bxi_block(const BatchSim *__restrict__ sims,
+                                          BatchState *__restrict__ st,
+                                          const int *__restrict__ map, int vb, float2 *sm)
 {
-    extern __shared__ float2 sm[];
-    const int s = map[blockIdx.x];
+    const int s = map[vb];
     const BatchSim &S = sims[s];
     BatchState &Z = st[s];
     if (!Z.active || (STAGE == 0 && !Z.need_stage0)) return;
-    const long row0 = (long)(blockIdx.x - S.first[BP_XI]) * S.tile[BP_XI];
+    const long row0 = (long)(vb - S.first[BP_XI]) * S.tile[BP_XI];
     EpilogueIO io{};
     if (STAGE == 0) {
         io.m = S.mbuf[Z.cur];
@@ -163,13 +163,26 @@ bxi_kernel(const BatchSim *__restrict__ sims, BatchState *__restrict__ st,
     }
 }
 
-// RK12Solver.Step's controller (solve_rk12.go:124-214) after one try, per
+#define MMX_BATCH_KERNEL(NAME, BODY, STATE_T)                                                \
+    template <int STAGE>                                                                     \
+    __global__ void __launch_bounds__(kBlock)                                                \
+    NAME(const BatchSim *__restrict__ sims, STATE_T *__restrict__ st,                        \
+         const int *__restrict__ map)                                                        \
+    {                                                                                        \
+        extern __shared__ float2 sm[];                                                       \
+        BODY<STAGE>(sims, st, map, blockIdx.x, sm);                                          \
+    }
+MMX_BATCH_KERNEL(bxf_kernel, bxf_block, const BatchState)
+MMX_BATCH_KERNEL(by_kernel, by_block, const BatchState)
+MMX_BATCH_KERNEL(bz_kernel, bz_block, const BatchState)
+MMX_BATCH_KERNEL(byi_kernel, byi_block, const BatchState)
+MMX_BATCH_KERNEL(bxi_kernel, bxi_block, BatchState)
+#undef MMX_BATCH_KERNEL
+
+// RK12Solver.Step's controller (solve_rk12.go:124-214) after one try, for one
 // simulation; mirrors the host loop of mmx_sim_step.
-__global__ void __launch_bounds__(64) bcontrol_kernel(BatchState *__restrict__ st, int nsim)
+__device__ __forceinline__ void control_one(BatchState &Z)
 {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= nsim) return;
-    BatchState &Z = st[s];
     if (!Z.active) return;
     const int maxTry = 10;
     const double headRoom = 0.8;
@@ -220,6 +233,110 @@ __global__ void __launch_bounds__(64) bcontrol_kernel(BatchState *__restrict__ s
     Z.h = (float)(Z.dt * Z.gamma);
 }
 
+__global__ void __launch_bounds__(64) bcontrol_kernel(BatchState *__restrict__ st, int nsim)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < nsim) control_one(st[s]);
+}
+
+// ---------------------------------------------------- the persistent launch
+//
+// One cooperative launch (every workgroup resident) runs up to `tries` tries
+// of the whole batch: the ten passes of a try and the controller one after
+// another, separated by grid-wide barriers, each pass's virtual blocks dealt
+// round-robin to the workgroups.  It stops early once no simulation is
+// active.  A barrier wait is bounded: past kSpinLimit polls the workgroup
+// records the time-out in `fail` and leaves; a timed-out barrier leaves the
+// barrier words inconsistent, so the host resets them before the next launch.
+
+struct PersistArgs {
+    const int *map[BP_N];
+    int nblk[BP_N];
+    int nsim;
+    int tries;
+    unsigned int *bar;   // [0]: arrivals, [1]: generation
+    unsigned int *fail;  // nonzero: a barrier wait timed out
+    int *nactive;        // active simulations after the last controller run
+};
+
+constexpr long kSpinLimit = 1L << 25;
+
+// Grid-wide barrier with agent-scope release/acquire (the workgroups of the
+// 8 XCDs do not share an L2).  Returns false when the wait timed out.
+__device__ bool grid_barrier(const PersistArgs &a)
+{
+    __shared__ int ok;
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const unsigned int gen =
+            __hip_atomic_load(&a.bar[1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+        const unsigned int arrived =
+            __hip_atomic_fetch_add(&a.bar[0], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        ok = 1;
+        if (arrived == gridDim.x - 1) {
+            __hip_atomic_store(&a.bar[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_fetch_add(&a.bar[1], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            long n = 0;
+            while (__hip_atomic_load(&a.bar[1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) ==
+                   gen) {
+                __builtin_amdgcn_s_sleep(2);
+                if (++n > kSpinLimit) {
+                    __hip_atomic_store(a.fail, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    ok = 0;
+                    break;
+                }
+            }
+        }
+    }
+    __syncthreads();
+    __threadfence();
+    return ok != 0;
+}
+
+template <int STAGE>
+__device__ bool persist_passes(const BatchSim *__restrict__ sims, BatchState *__restrict__ st,
+                               const PersistArgs &a, float2 *sm)
+{
+#define MMX_PASS(P, BODY)                                                                    \
+    if (a.nblk[P] > 0) {                                                                     \
+        for (int vb = blockIdx.x; vb < a.nblk[P]; vb += gridDim.x) {                         \
+            BODY<STAGE>(sims, st, a.map[P], vb, sm);                                         \
+            __syncthreads(); /* the next tile reuses the LDS */                              \
+        }                                                                                    \
+        if (!grid_barrier(a)) return false;                                                  \
+    }
+    MMX_PASS(BP_XF, bxf_block)
+    MMX_PASS(BP_Y, by_block)
+    MMX_PASS(BP_Z, bz_block)
+    MMX_PASS(BP_YI, byi_block)
+    MMX_PASS(BP_XI, bxi_block)
+#undef MMX_PASS
+    return true;
+}
+
+__global__ void __launch_bounds__(kBlock)
+bpersist_kernel(const BatchSim *__restrict__ sims, BatchState *__restrict__ st, PersistArgs a)
+{
+    extern __shared__ float2 sm[];
+    for (int k = 0; k < a.tries; k++) {
+        if (!persist_passes<0>(sims, st, a, sm)) return;
+        if (!persist_passes<1>(sims, st, a, sm)) return;
+        if (blockIdx.x == 0) {
+            for (int s = threadIdx.x; s < a.nsim; s += blockDim.x) control_one(st[s]);
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                int n = 0;
+                for (int s = 0; s < a.nsim; s++) n += st[s].active;
+                __hip_atomic_store(a.nactive, n, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+        }
+        if (!grid_barrier(a)) return;
+        if (__hip_atomic_load(a.nactive, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) return;
+    }
+}
+
 // ------------------------------------------------------------ host side
 
 struct BatchImpl {
@@ -237,6 +354,8 @@ struct BatchImpl {
     hipGraph_t graph = nullptr;
     int graph_tries = 0;        // tries recorded in the graph (0: none yet)
     int tries_per_graph = 8;    // 0: eager launches
+    int persist_tries = 0;      // > 0: one persistent launch per this many tries
+    unsigned int *dpersist = nullptr; // barrier words, time-out flag, active count
     bool tables_dirty = true;
 };
 
@@ -476,6 +595,50 @@ int ensure_graph(BatchImpl &B)
     return MMX_OK;
 }
 
+// Up to B.persist_tries tries of the whole batch in one cooperative launch
+// (bpersist_kernel); the grid is what can be resident at once, at most the
+// largest pass's block count.
+int persist_launch(BatchImpl &B)
+{
+    size_t lds = 0;
+    int maxblk = 1;
+    PersistArgs a{};
+    for (int p = 0; p < BP_N; p++) {
+        lds = std::max(lds, B.lds[p]);
+        maxblk = std::max(maxblk, B.nblk[p]);
+        a.map[p] = B.dmap[p];
+        a.nblk[p] = B.nblk[p];
+    }
+    a.nsim = (int)B.sims.size();
+    a.tries = B.persist_tries;
+    a.bar = B.dpersist;
+    a.fail = B.dpersist + 2;
+    a.nactive = reinterpret_cast<int *>(B.dpersist + 3);
+    allow_lds_batch(bpersist_kernel, lds);
+    int dev = 0, coop = 0, ncu = 0, per_cu = 0;
+    MMX_HIP(hipGetDevice(&dev));
+    MMX_HIP(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev));
+    MMX_REQUIRE(coop, MMX_ERR_STATE, "batch: the device has no cooperative launch");
+    MMX_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    MMX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, bpersist_kernel, kBlock, lds));
+    MMX_REQUIRE(per_cu > 0, MMX_ERR_SIZE, "batch: the persistent kernel does not fit a CU (%zu B LDS)",
+                lds);
+    const int grid = std::max(1, std::min(per_cu * ncu, maxblk));
+    // barrier words, time-out flag and active count start at zero (a timed-out
+    // launch may leave them inconsistent)
+    MMX_HIP(hipMemsetAsync(B.dpersist, 0, 4 * sizeof(unsigned int), B.st));
+    const BatchSim *sims = B.dsims;
+    BatchState *st = B.dst;
+    void *args[] = {&sims, &st, &a};
+    MMX_HIP(hipLaunchCooperativeKernel(reinterpret_cast<const void *>(bpersist_kernel),
+                                       dim3(grid), dim3(kBlock), args, (unsigned int)lds, B.st));
+    unsigned int h[4];
+    MMX_HIP(hipMemcpyAsync(h, B.dpersist, sizeof(h), hipMemcpyDeviceToHost, B.st));
+    MMX_HIP(hipStreamSynchronize(B.st));
+    MMX_REQUIRE(h[2] == 0, MMX_ERR_HIP, "batch: a grid barrier of the persistent launch timed out");
+    return MMX_OK;
+}
+
 // state back to the host; optionally into the simulations
 int fetch_state(BatchImpl &B, bool into_sims, int *nactive)
 {
@@ -528,7 +691,10 @@ int batch_advance(BatchImpl &B, double duration, long long nsteps)
     long long tries = 0;
     int rc = MMX_OK;
     while (nactive > 0 && rc == MMX_OK) {
-        if (B.tries_per_graph > 0) {
+        if (B.persist_tries > 0) {
+            rc = persist_launch(B);
+            tries += B.persist_tries;
+        } else if (B.tries_per_graph > 0) {
             rc = ensure_graph(B);
             if (rc == MMX_OK && hipGraphLaunch(B.gexec, B.st) != hipSuccess) {
                 set_error("batch: graph launch failed");
@@ -570,6 +736,7 @@ int mmx_batch_create(mmx_batch **batch, mmx_sim *const *sims, int nsim)
     int rc = MMX_OK;
     if (hipMalloc(&B.dsims, sizeof(BatchSim) * nsim) != hipSuccess ||
         hipMalloc(&B.dst, sizeof(BatchState) * nsim) != hipSuccess ||
+        hipMalloc(&B.dpersist, 4 * sizeof(unsigned int)) != hipSuccess ||
         hipStreamCreateWithFlags(&B.st, hipStreamNonBlocking) != hipSuccess) {
         set_error("mmx_batch_create: cannot allocate the batch tables");
         rc = MMX_ERR_NOMEM;
@@ -592,6 +759,7 @@ int mmx_batch_destroy(mmx_batch *batch)
     for (int p = 0; p < BP_N; p++) (void)hipFree(B.dmap[p]);
     (void)hipFree(B.dsims);
     (void)hipFree(B.dst);
+    (void)hipFree(B.dpersist);
     if (B.st) (void)hipStreamDestroy(B.st);
     delete batch;
     return MMX_OK;
@@ -603,6 +771,14 @@ int mmx_batch_set_graph(mmx_batch *batch, int tries_per_graph)
                 "mmx_batch_set_graph: tries per graph must be 0..256");
     batch->impl.tries_per_graph = tries_per_graph;
     drop_graph(batch->impl);
+    return MMX_OK;
+}
+
+int mmx_batch_set_persistent(mmx_batch *batch, int tries_per_launch)
+{
+    MMX_REQUIRE(batch && tries_per_launch >= 0 && tries_per_launch <= 4096, MMX_ERR_ARG,
+                "mmx_batch_set_persistent: tries per launch must be 0..4096");
+    batch->impl.persist_tries = tries_per_launch;
     return MMX_OK;
 }
 

@@ -80,6 +80,7 @@ _SIGS = {
     "mmx_batch_run": (_i, [_vp, _d]),
     "mmx_batch_steps": (_i, [_vp, _i]),
     "mmx_batch_set_graph": (_i, [_vp, _i]),
+    "mmx_batch_set_persistent": (_i, [_vp, _i]),
     "mmx_batch_tune": (_i, [_vp, _i, C.POINTER(_i)]),
     "mmx_batch_set_lds": (_i, [_vp, C.POINTER(_i)]),
     "mmx_batch_info": (_i, [_vp, C.POINTER(_i), C.POINTER(_i)]),

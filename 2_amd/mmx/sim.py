@@ -226,6 +226,10 @@ class Batch:
     def set_graph(self, tries_per_graph):
         check(lib().mmx_batch_set_graph(self._h, int(tries_per_graph)))
 
+    def set_persistent(self, tries_per_launch):
+        """One persistent launch per `tries_per_launch` tries (0: off)."""
+        check(lib().mmx_batch_set_persistent(self._h, int(tries_per_launch)))
+
     def run(self, duration):
         check(lib().mmx_batch_run(self._h, float(duration)))
 

@@ -397,6 +397,11 @@ int mmx_batch_run(mmx_batch *batch, double duration);
 int mmx_batch_steps(mmx_batch *batch, int nsteps);
 /* Tries per graph replay (default 8); 0 = plain launches, no graph. */
 int mmx_batch_set_graph(mmx_batch *batch, int tries_per_graph);
+/* One persistent launch per `tries_per_launch` tries (0: off, the graph or
+ * plain launches above): a cooperative launch whose workgroups run every pass
+ * of every simulation and the step controller, separated by grid-wide
+ * barriers, and stop early when no simulation is active. */
+int mmx_batch_set_persistent(mmx_batch *batch, int tries_per_launch);
 /* Autotuner: for each of the 5 passes (x forward, y forward or film y,
  * z, y inverse, x inverse) times the pass over the batch at LDS budgets of
  * 8..128 KB per workgroup (every simulation's tile shape follows from the

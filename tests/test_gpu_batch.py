@@ -5,6 +5,8 @@
   oracle's RK12 tries (orc_rk12_try), 1e-5 after 6 steps;
 * the hipGraph replays against plain launches of the same kernels: bit-exact
   after every replay, with eager work (reductions) between the replays;
+* the persistent launch (every pass and the controller in one cooperative
+  launch, grid-wide barriers) against plain launches: bit-exact;
 * tile shapes (the autotuner's LDS budgets) do not change the arithmetic:
   bit-exact across budgets;
 * free runs against each simulation's own engine (mmx_sim_run) within the
@@ -112,6 +114,25 @@ def test_graph_replays_bit_exact(mmx_gpu):
         _say("graph replay", rep, sb[0].getv("t"))
 
 
+@pytest.mark.parametrize("tries", [1, 3, 64])
+def test_persistent_launch_bit_exact(mmx_gpu, tries):
+    """One cooperative launch per `tries` tries (passes and controller behind
+    grid barriers) against plain launches: same bits, step counts and times,
+    across steps() and run() stops that fall inside a launch."""
+    mmx = mmx_gpu
+    sa, _ = _make(mmx)
+    sb, _ = _make(mmx)
+    ba = mmx.Batch(sa, tries_per_graph=0)
+    bb = mmx.Batch(sb)
+    bb.set_persistent(tries)
+    for rep, (kind, amount) in enumerate((("steps", 5), ("run", 2e-13), ("steps", 7))):
+        getattr(ba, kind)(amount)
+        getattr(bb, kind)(amount)
+        for x, y in zip(_snapshot(sa), _snapshot(sb)):
+            assert np.array_equal(x[0], y[0]) and x[1:] == y[1:], (rep, kind)
+        _say("persistent", tries, kind, sb[0].getv("step"), sb[0].getv("t"))
+
+
 def test_tile_budgets_do_not_change_results(mmx_gpu):
     mmx = mmx_gpu
     sa, _ = _make(mmx)
@@ -151,3 +172,5 @@ def test_batch_errors(mmx_gpu):
         b.set_lds({"xf": 0, "y": 8, "z": 8, "yi": 8, "xi": 8})
     with pytest.raises(mmx.MMXError):
         b.set_graph(-1)
+    with pytest.raises(mmx.MMXError):
+        b.set_persistent(-1)
