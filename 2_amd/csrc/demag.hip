@@ -233,7 +233,7 @@ int demag_inverse(DemagPlanImpl *P, int mode, const FieldParams &fp, const Epilo
     const int nb = div_up(g.nrows, P->tiles.rows_x);
     const int RB = P->tiles.rows_x;
     const int pass_xi = (g.p0 == 1) ? 2 : 4;
-    if (P->fast_x && fast_xi_applies(fp)) {
+    if (P->fast_x) {
         MMX_TRY(fast_xi(P, mode, fp, io, st));
         mark(P, st, pass_xi);
         return MMX_OK;

@@ -1,738 +1,12 @@
-// demag_fast.hip -- the demag passes for power-of-two transform lengths.
-//
-// Same five passes as demag.hip (xf, yf, z, yi, xi; films: xf, y2d, xi) with
-// the FFT held in registers (four-step split, reg_fft.hpp): each pass reads its
-// input once and writes its output once, with one LDS transpose per transform
-// direction.  Strided passes work on tiles of W spectrum columns (W >= 16, so
-// each row segment is at least 128 bytes on 128-byte aligned rows); blocks are
-// mapped to tiles XCD by XCD (xcd_tile).
-#include "epilogue.hpp"
-#include "reg_fft.hpp"
+// demag_fast.hip -- host launchers of the demag passes for power-of-two
+// transform lengths (the kernels are in fast_kernels.hpp; the general xi
+// epilogue is instantiated in xi_gen.hip).
+#include "fast_kernels.hpp"
 
 namespace mmx {
 
-// Work split of a strided (y or z) pass: T threads per line, E registers, W
-// lines (spectrum columns) per workgroup, LDS row pitch S.
-template <int L> struct ColTile {
-    static constexpr int T = Split<L>::T;
-    static constexpr int E = Split<L>::E;
-    static constexpr int G = E / T;
-    // length 2048: 2 columns (3 % faster than 4 at 64x2048x2048: 49 KB of LDS
-    // per workgroup instead of 83, three workgroups per CU instead of one)
-    static constexpr int W = (L == 2048) ? 2 : (L == 1024 ? 8 : 256 / T);
-    static constexpr int S = lds_pitch(T, W, G);
-    static constexpr int NT = T * W;                     // threads
-    static constexpr int LDS = (E * S + L) * 8;           // transpose + twiddles
-};
-
-// Row pass (x): R rows per workgroup, T threads per row.
-template <int LX> struct RowTile {
-    static constexpr int T = Split<LX>::T;
-    static constexpr int E = Split<LX>::E;
-    static constexpr int G = E / T;
-    static constexpr int R = (LX == 2048) ? 4 : (LX == 1024 ? 8 : 256 / T);
-    // pitch = 1 mod 32 complex: the transposed reads of 16-32 rows x T lanes
-    // hit distinct bank pairs
-    static constexpr int S = T * R + 1;
-    static constexpr int NT = T * R;
-    static constexpr int LDS = (E * S + 2 * LX + 1) * 8;
-};
-
-// ---------------------------------------------------- column transforms
-//
-// Thread (w, t) of a column tile: w = tid % W (column), t = tid / W.
-// Transpose buffer index of Y[t][k1] of column w: k1*S + t*W + w.
-
-template <int L, int W, int S, bool HALFIN = false>
-__device__ __forceinline__ void col_forward(float2 (&v)[Split<L>::E],
-                                            float2 (&u)[Split<L>::E / Split<L>::T][Split<L>::T],
-                                            int w, int t, float2 *sbuf, const float2 *stw)
-{
-    constexpr int T = Split<L>::T, E = Split<L>::E, G = E / T;
-    fs_first<L, -1, HALFIN>(v, t, stw);
-#pragma unroll
-    for (int k1 = 0; k1 < E; k1++) sbuf[k1 * S + t * W + w] = v[k1];
-    __syncthreads();
-#pragma unroll
-    for (int gg = 0; gg < G; gg++) {
-#pragma unroll
-        for (int t2 = 0; t2 < T; t2++) u[gg][t2] = sbuf[(t * G + gg) * S + t2 * W + w];
-        fft_reg<T, -1>(u[gg]);
-    }
-    __syncthreads();
-}
-
-template <int L, int W, int S>
-__device__ __forceinline__ void col_inverse(float2 (&u)[Split<L>::E / Split<L>::T][Split<L>::T],
-                                            float2 (&v)[Split<L>::E], int w, int t,
-                                            float2 *sbuf, const float2 *stw)
-{
-    constexpr int T = Split<L>::T, E = Split<L>::E, G = E / T;
-#pragma unroll
-    for (int gg = 0; gg < G; gg++) {
-        fft_reg<T, +1>(u[gg]);
-        const int k1 = t * G + gg;
-#pragma unroll
-        for (int t2 = 0; t2 < T; t2++) {
-            const float2 x = (t2 == 0) ? u[gg][0] : cmulc(u[gg][t2], stw[t2 * k1]);
-            sbuf[k1 * S + t2 * W + w] = x;
-        }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int k1 = 0; k1 < E; k1++) v[k1] = sbuf[k1 * S + t * W + w];
-    fft_reg<E, +1>(v);
-    __syncthreads();
-}
-
-// *(T *)(base + off) with a 32-bit byte offset: with a wave-uniform base this
-// is one global access with a scalar base and a vector offset.
-template <typename T> __device__ __forceinline__ T ld_off(const void *base, unsigned off)
-{
-    return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + off);
-}
-template <typename T> __device__ __forceinline__ void st_off(void *base, unsigned off, T v)
-{
-    *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + off) = v;
-}
-
-// yf: FFT along y of the n0 nonzero planes.  FULL: periodic y (n1 == L);
-// otherwise n1 == L/2 and rows n1 .. L-1 are zero padding (not read).
-template <int L, bool FULL>
-__global__ void __launch_bounds__(ColTile<L>::NT)
-yf_fast(DemagGeom g, int ntx, const float2 *__restrict__ tw, const float2 *__restrict__ xb,
-        float2 *__restrict__ yb)
-{
-    using C = ColTile<L>;
-    constexpr int NQ = FULL ? C::E : C::E / 2; // rows t + T q held, q < NQ
-    extern __shared__ float2 sm[];
-    float2 *sbuf = sm, *stw = sm + C::E * C::S;
-    const int tile = xcd_tile(blockIdx.x, gridDim.x);
-    const int tx = tile % ntx, rest = tile / ntx, i = rest % g.n0, c = rest / g.n0;
-    const int kx0 = tx * C::W;
-    const int w = threadIdx.x % C::W, t = threadIdx.x / C::W;
-    for (int q = threadIdx.x; q < L; q += C::NT) stw[q] = tw[q];
-    const bool ok = kx0 + w < g.h2;
-    // columns past h2 load column h2 - 1 (in bounds) and store nothing
-    const int col = ok ? kx0 + w : g.h2 - 1;
-    const float2 *src = xb + ((size_t)c * g.nrows + (size_t)i * g.n1) * g.sx;
-    const unsigned roff = (unsigned)(((size_t)t * g.sx + col) * sizeof(float2));
-    float2 v[C::E];
-#pragma unroll
-    for (int q = 0; q < NQ; q++) v[q] = ld_off<float2>(src + (size_t)q * C::T * g.sx, roff);
-    __syncthreads();
-    float2 u[C::G][C::T];
-    col_forward<L, C::W, C::S, !FULL>(v, u, w, t, sbuf, stw);
-    if (!ok) return;
-    float2 *dst = yb + ((size_t)c * g.n0 + i) * (size_t)g.p1 * g.sx;
-    const unsigned woff = (unsigned)(((size_t)t * C::G * g.sx + col) * sizeof(float2));
-#pragma unroll
-    for (int gg = 0; gg < C::G; gg++)
-#pragma unroll
-        for (int k2 = 0; k2 < C::T; k2++)
-            st_off<float2>(dst + (size_t)(gg + C::E * k2) * g.sx, woff, u[gg][k2]);
-}
-
-// yi: inverse FFT along y, keep j < n1 (FULL: periodic y, n1 == L)
-template <int L, bool FULL>
-__global__ void __launch_bounds__(ColTile<L>::NT)
-yi_fast(DemagGeom g, int ntx, const float2 *__restrict__ tw, const float2 *__restrict__ yb,
-        float2 *__restrict__ xb)
-{
-    using C = ColTile<L>;
-    constexpr int NQ = FULL ? C::E : C::E / 2;
-    extern __shared__ float2 sm[];
-    float2 *sbuf = sm, *stw = sm + C::E * C::S;
-    const int tile = xcd_tile(blockIdx.x, gridDim.x);
-    const int tx = tile % ntx, rest = tile / ntx, i = rest % g.n0, c = rest / g.n0;
-    const int kx0 = tx * C::W;
-    const int w = threadIdx.x % C::W, t = threadIdx.x / C::W;
-    for (int q = threadIdx.x; q < L; q += C::NT) stw[q] = tw[q];
-    const bool ok = kx0 + w < g.h2;
-    const int col = ok ? kx0 + w : g.h2 - 1;
-    const float2 *src = yb + ((size_t)c * g.n0 + i) * (size_t)g.p1 * g.sx;
-    const unsigned roff = (unsigned)(((size_t)t * C::G * g.sx + col) * sizeof(float2));
-    float2 u[C::G][C::T];
-#pragma unroll
-    for (int gg = 0; gg < C::G; gg++)
-#pragma unroll
-        for (int k2 = 0; k2 < C::T; k2++)
-            u[gg][k2] = ld_off<float2>(src + (size_t)(gg + C::E * k2) * g.sx, roff);
-    __syncthreads();
-    float2 v[C::E];
-    col_inverse<L, C::W, C::S>(u, v, w, t, sbuf, stw);
-    if (!ok) return;
-    float2 *dst = xb + ((size_t)c * g.nrows + (size_t)i * g.n1) * g.sx;
-    const unsigned woff = (unsigned)(((size_t)t * g.sx + col) * sizeof(float2));
-#pragma unroll
-    for (int q = 0; q < NQ; q++) st_off<float2>(dst + (size_t)q * C::T * g.sx, woff, v[q]);
-}
-
-// z pass: three thread groups, one per field component, each transforms its
-// component of W spectrum columns along z; the 3x3 kernel multiply runs
-// through LDS, one octant kernel point for each pair kz, p0-kz.
-template <int L> struct ZTile {
-    static constexpr int T = Split<L>::T;
-    static constexpr int E = Split<L>::E;
-    static constexpr int G = E / T;
-    // 8 columns for L = 128 (measured: 141 us vs 145 us with 16 at 256x256x64)
-    static constexpr int W = (L == 2048) ? 2 : (L == 128 ? 8 : 128 / T);
-    static constexpr int S = lds_pitch(T, W, G);
-    static constexpr int NG = T * W;          // threads per component group
-    static constexpr int NT = 3 * NG;
-    static constexpr int LDS = (3 * E * S + L) * 8;
-    // spectrum rows (one kz, W columns) are exchanged by lanes (t, w) at rows
-    // t*G + g: with G = 2 only every other row's banks would be used, so bit
-    // 0 of the row is flipped by the row bit that advances one bank span
-    static constexpr int SWZ = (G == 2 && W < 32) ? __builtin_ctz(32 / W) : -1;
-    static_assert(NG % 64 == 0, "component groups must be whole waves");
-    static_assert(NT % W == 0, "");
-};
-
-template <int L> __device__ __forceinline__ int zrow(int kz)
-{
-    if constexpr (ZTile<L>::SWZ >= 0)
-        return kz ^ ((kz >> ZTile<L>::SWZ) & 1);
-    else
-        return kz;
-}
-
-// ky visiting order 0, p/2, 1, p-1, 2, p-2, ...: tiles of ky and p - ky
-// (which read the same kernel octant rows) are neighbours
-__device__ __forceinline__ int mirror_order(int i, int p)
-{
-    if (i == 0) return 0;
-    if (p & 1) { // odd p: pairs (u, p-u) for u = 1 .. (p-1)/2
-        const int u = (i - 1) / 2 + 1;
-        return (i & 1) ? u : p - u;
-    }
-    if (i == 1) return p / 2;
-    const int u = (i - 2) / 2 + 1;
-    return (i & 1) ? p - u : u;
-}
-
-// FULL: periodic z (n0 == L, every plane holds data); otherwise n0 == L/2
-// and planes n0 .. L-1 are zero padding (not read, not written).  Byte
-// offsets from the wave-uniform bases stay below 4 GiB (fast_z_addressable).
-template <int L, bool FULL>
-__global__ void __launch_bounds__(ZTile<L>::NT)
-z_fast(DemagGeom g, int ntx, const float2 *__restrict__ tw, const float *__restrict__ K,
-       float2 *__restrict__ yb)
-{
-    using Z = ZTile<L>;
-    constexpr int NQ = FULL ? Z::E : Z::E / 2; // planes t + T q held, q < NQ
-    extern __shared__ float2 sm[];
-    float2 *stw = sm + 3 * Z::E * Z::S;
-    const int tile = xcd_tile(blockIdx.x, gridDim.x);
-    const int tx = tile % ntx, ky = mirror_order(tile / ntx, g.p1);
-    const int kx0 = tx * Z::W;
-    const int c = __builtin_amdgcn_readfirstlane(threadIdx.x / Z::NG);
-    const int lt = threadIdx.x % Z::NG;
-    const int w = lt % Z::W, t = lt / Z::W;
-    float2 *sbuf = sm + c * Z::E * Z::S;
-    for (int q = threadIdx.x; q < L; q += Z::NT) stw[q] = tw[q];
-    const bool ok = kx0 + w < g.h2;
-    // columns past h2 load column h2 - 1 (in bounds) and store nothing
-    const int col = ok ? kx0 + w : g.h2 - 1;
-    const size_t istride = (size_t)g.p1 * g.sx;
-    float2 *cbase = yb + (size_t)c * g.n0 * istride + (size_t)ky * g.sx;
-    const unsigned voff = (unsigned)(((size_t)t * istride + col) * sizeof(float2));
-    const bool my = ky > g.p1 / 2;
-    const int kyp = my ? g.p1 - ky : ky;
-    const float sy = my ? -1.0f : 1.0f;
-    // the kernel octant points this thread multiplies, loaded before the
-    // spectrum so that their latency hides behind the loads and the forward
-    // transform (when they fit in a few registers)
-    constexpr int NITEM = ((L / 2 + 1) * Z::W + Z::NT - 1) / Z::NT;
-    constexpr int NPF = (NITEM <= 4) ? NITEM : 1;
-    float2 kpf[NPF][3];
-    const float *kbase = K + 6 * ((size_t)kyp * g.h2 + kx0);
-    const size_t kzs = (size_t)g.ko1 * g.h2;   // octant points per kz
-    if constexpr (NITEM <= 4) {
-        const int w2 = threadIdx.x % Z::W;
-        const unsigned koff = (unsigned)(((threadIdx.x / Z::W) * kzs + w2) * 6 * sizeof(float));
-#pragma unroll
-        for (int r = 0; r < NITEM; r++) {
-            const int item = threadIdx.x + r * Z::NT;
-            const float *kb = kbase + 6 * (size_t)(r * (Z::NT / Z::W)) * kzs;
-            if (item < (L / 2 + 1) * Z::W && kx0 + w2 < g.h2) {
-                kpf[r][0] = ld_off<float2>(kb, koff);
-                kpf[r][1] = ld_off<float2>(kb, koff + 8);
-                kpf[r][2] = ld_off<float2>(kb, koff + 16);
-            } else {
-                kpf[r][0] = kpf[r][1] = kpf[r][2] = float2{0.0f, 0.0f};
-            }
-        }
-    }
-    float2 v[Z::E];
-#pragma unroll
-    for (int q = 0; q < NQ; q++) v[q] = ld_off<float2>(cbase + (size_t)q * Z::T * istride, voff);
-    __syncthreads();
-    float2 u[Z::G][Z::T];
-    col_forward<L, Z::W, Z::S, !FULL>(v, u, w, t, sbuf, stw);
-
-    // spectrum of all three components through LDS: spec[c][zrow(kz)][w]
-#pragma unroll
-    for (int gg = 0; gg < Z::G; gg++)
-#pragma unroll
-        for (int k2 = 0; k2 < Z::T; k2++)
-            sm[(c * L + zrow<L>(t * Z::G + gg + Z::E * k2)) * Z::W + w] = u[gg][k2];
-    __syncthreads();
-    auto multiply = [&](const Kern6 &k, int kzp, int w2) {
-        const int r1 = zrow<L>(kzp);
-        kern_apply(k, sm[(0 * L + r1) * Z::W + w2], sm[(1 * L + r1) * Z::W + w2],
-                   sm[(2 * L + r1) * Z::W + w2]);
-        if (kzp > 0 && kzp < L / 2) {
-            Kern6 km = k; // K at p0 - kz: odd in z for XZ and XY
-            km.xz = -km.xz;
-            km.xy = -km.xy;
-            const int r2 = zrow<L>(L - kzp);
-            kern_apply(km, sm[(0 * L + r2) * Z::W + w2], sm[(1 * L + r2) * Z::W + w2],
-                       sm[(2 * L + r2) * Z::W + w2]);
-        }
-    };
-    if constexpr (NITEM <= 4) {
-#pragma unroll
-        for (int r = 0; r < NITEM; r++) {
-            const int item = threadIdx.x + r * Z::NT;
-            const int w2 = item % Z::W, kzp = item / Z::W;
-            if (item >= (L / 2 + 1) * Z::W || kx0 + w2 >= g.h2) continue;
-            // the layout of kern_octant (XX,YY | ZZ,YZ | XZ,XY), z sign +1
-            Kern6 k;
-            k.xx = kpf[r][0].x;
-            k.yy = kpf[r][0].y;
-            k.zz = kpf[r][1].x;
-            k.yz = sy * kpf[r][1].y;
-            k.xz = kpf[r][2].x;
-            k.xy = sy * kpf[r][2].y;
-            multiply(k, kzp, w2);
-        }
-    } else {
-        for (int item = threadIdx.x; item < (L / 2 + 1) * Z::W; item += Z::NT) {
-            const int w2 = item % Z::W, kzp = item / Z::W;
-            if (kx0 + w2 >= g.h2) continue;
-            multiply(kern_octant(g, K, kzp, kyp, kx0 + w2, 1.0f, sy), kzp, w2);
-        }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int gg = 0; gg < Z::G; gg++)
-#pragma unroll
-        for (int k2 = 0; k2 < Z::T; k2++)
-            u[gg][k2] = sm[(c * L + zrow<L>(t * Z::G + gg + Z::E * k2)) * Z::W + w];
-    __syncthreads();
-    col_inverse<L, Z::W, Z::S>(u, v, w, t, sbuf, stw);
-    if (ok) {
-#pragma unroll
-        for (int q = 0; q < NQ; q++) st_off<float2>(cbase + (size_t)q * Z::T * istride, voff, v[q]);
-    }
-}
-
-// films (p0 == 1): y transform, kernel multiply and inverse y in one kernel
-// (FULL: periodic y, n1 == L; otherwise rows n1 .. L-1 are zero padding)
-template <int L, bool FULL>
-__global__ void __launch_bounds__(ColTile<L>::NT)
-y2d_fast(DemagGeom g, int ntx, const float2 *__restrict__ tw, const float *__restrict__ K,
-         float2 *__restrict__ xb)
-{
-    using C = ColTile<L>;
-    constexpr int NQ = FULL ? C::E : C::E / 2;
-    extern __shared__ float2 sm[];
-    float2 *sbuf = sm, *stw = sm + C::E * C::S;
-    const int tile = xcd_tile(blockIdx.x, gridDim.x);
-    const int kx0 = tile * C::W;
-    const int w = threadIdx.x % C::W, t = threadIdx.x / C::W;
-    for (int q = threadIdx.x; q < L; q += C::NT) stw[q] = tw[q];
-    const bool ok = kx0 + w < g.h2;
-    const int col = ok ? kx0 + w : g.h2 - 1;
-    const size_t cstride = (size_t)g.nrows * g.sx;
-    const unsigned off = (unsigned)(((size_t)t * g.sx + col) * sizeof(float2));
-    float2 u[3][C::G][C::T];
-    __syncthreads();
-#pragma unroll
-    for (int c = 0; c < 3; c++) {
-        float2 v[C::E];
-#pragma unroll
-        for (int q = 0; q < NQ; q++)
-            v[q] = ld_off<float2>(xb + c * cstride + (size_t)q * C::T * g.sx, off);
-        col_forward<L, C::W, C::S, !FULL>(v, u[c], w, t, sbuf, stw);
-    }
-    if (ok) {
-#pragma unroll
-        for (int gg = 0; gg < C::G; gg++)
-#pragma unroll
-            for (int k2 = 0; k2 < C::T; k2++) {
-                const Kern6 k = kern_at(g, K, 0, t * C::G + gg + C::E * k2, kx0 + w);
-                kern_apply(k, u[0][gg][k2], u[1][gg][k2], u[2][gg][k2]);
-            }
-    }
-#pragma unroll
-    for (int c = 0; c < 3; c++) {
-        float2 v[C::E];
-        col_inverse<L, C::W, C::S>(u[c], v, w, t, sbuf, stw);
-        if (ok) {
-#pragma unroll
-            for (int q = 0; q < NQ; q++)
-                st_off<float2>(xb + c * cstride + (size_t)q * C::T * g.sx, off, v[q]);
-        }
-    }
-}
-
-// ---------------------------------------------------------- row transforms
-//
-// Thread (r, t): r = tid / T (row of the tile), t = tid % T.  Transpose
-// buffer index of Y[t][k1] of row r: k1*S + r*T + t.
-
-template <int LX, bool HALFIN = false>
-__device__ __forceinline__ void row_forward(float2 (&v)[RowTile<LX>::E],
-                                            float2 (&u)[RowTile<LX>::G][RowTile<LX>::T], int r,
-                                            int t, float2 *sbuf, const float2 *stw)
-{
-    using C = RowTile<LX>;
-    fs_first<LX, -1, HALFIN>(v, t, stw);
-#pragma unroll
-    for (int k1 = 0; k1 < C::E; k1++) sbuf[k1 * C::S + r * C::T + t] = v[k1];
-    __syncthreads();
-#pragma unroll
-    for (int gg = 0; gg < C::G; gg++) {
-#pragma unroll
-        for (int t2 = 0; t2 < C::T; t2++) u[gg][t2] = sbuf[(t * C::G + gg) * C::S + r * C::T + t2];
-        fft_reg<C::T, -1>(u[gg]);
-    }
-    __syncthreads();
-}
-
-template <int LX>
-__device__ __forceinline__ void row_inverse(float2 (&u)[RowTile<LX>::G][RowTile<LX>::T],
-                                            float2 (&v)[RowTile<LX>::E], int r, int t,
-                                            float2 *sbuf, const float2 *stw)
-{
-    using C = RowTile<LX>;
-#pragma unroll
-    for (int gg = 0; gg < C::G; gg++) {
-        fft_reg<C::T, +1>(u[gg]);
-        const int k1 = t * C::G + gg;
-#pragma unroll
-        for (int t2 = 0; t2 < C::T; t2++) {
-            const float2 x = (t2 == 0) ? u[gg][0] : cmulc(u[gg][t2], stw[t2 * k1]);
-            sbuf[k1 * C::S + r * C::T + t2] = x;
-        }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int k1 = 0; k1 < C::E; k1++) v[k1] = sbuf[k1 * C::S + r * C::T + t];
-    fft_reg<C::E, +1>(v);
-    __syncthreads();
-}
-
-__device__ __forceinline__ float2 shfl2(float2 a, int lane)
-{
-    return float2{__shfl(a.x, lane, 64), __shfl(a.y, lane, 64)};
-}
-
-// conj(Z[LX - k]) for the value u[g][k2] held at k = t G + g + E k2 (all lanes
-// of the wave must call this: it shuffles)
-template <int LX, int GG, int K2>
-__device__ __forceinline__ float2 mirror_conj(const float2 (&u)[RowTile<LX>::G][RowTile<LX>::T],
-                                              int t, int lane_row0)
-{
-    using C = RowTile<LX>;
-    constexpr int gsrc = (C::G - GG) % C::G;
-    const int tl = (GG == 0) ? ((C::T - t) % C::T) : (C::T - 1 - t);
-    float2 a = shfl2(u[gsrc][C::T - 1 - K2], lane_row0 + tl);
-    if (GG == 0 && t == 0) a = u[0][(C::T - K2) % C::T];
-    return cconj(a);
-}
-
-// xf: R2C along x of the n0*n1 data rows (half-length complex FFT).  FULL:
-// periodic x (n2 == 2 LX, every pair holds data); otherwise n2 == LX and the
-// pairs LX/2 .. LX-1 are zero padding (not read).  Lane offsets are 32-bit
-// from the component bases (fast_x_addressable).
-template <int LX, int PRED, bool FULL>
-__global__ void __launch_bounds__(RowTile<LX>::NT)
-xf_fast(DemagGeom g, const float2 *__restrict__ tw, const float2 *__restrict__ rtw,
-        const float *__restrict__ m, const float *__restrict__ msat_map,
-        const float *__restrict__ tau0, float h, const float *__restrict__ norm_map,
-        float *__restrict__ mpred, float2 *__restrict__ xb)
-{
-    using C = RowTile<LX>;
-    constexpr int NQ = FULL ? C::E : C::E / 2; // cell pairs 2 (t + T q), q < NQ
-    extern __shared__ float2 sm[];
-    float2 *sbuf = sm, *stw = sm + C::E * C::S, *srtw = stw + LX;
-    const int tile = xcd_tile(blockIdx.x, gridDim.x);
-    const int r = threadIdx.x / C::T, t = threadIdx.x % C::T;
-    const int row = tile * C::R + r;
-    const bool ok = row < g.nrows;
-    const int crow = ok ? row : (int)g.nrows - 1; // rows past the end load the last row
-    for (int q = threadIdx.x; q < LX; q += C::NT) stw[q] = tw[q];
-    for (int q = threadIdx.x; q <= LX; q += C::NT) srtw[q] = rtw[q];
-    const size_t N = g.N;
-    const unsigned cb = (unsigned)(crow * g.n2 + 2 * t) * 4u;
-    float2 v[3][C::E];
-#pragma unroll
-    for (int q = 0; q < NQ; q++) {
-        const unsigned o = cb + (unsigned)(q * 2 * C::T * 4);
-        float a0[2], a1[2], a2[2];
-        {
-            const float2 x0 = ld_off<float2>(m, o), x1 = ld_off<float2>(m + N, o),
-                         x2 = ld_off<float2>(m + 2 * N, o);
-            a0[0] = x0.x; a0[1] = x0.y;
-            a1[0] = x1.x; a1[1] = x1.y;
-            a2[0] = x2.x; a2[1] = x2.y;
-        }
-        if (PRED) {
-            const float2 p0 = ld_off<float2>(tau0, o), p1 = ld_off<float2>(tau0 + N, o),
-                         p2 = ld_off<float2>(tau0 + 2 * N, o);
-            const float2 nm = norm_map ? ld_off<float2>(norm_map, o) : float2{1.0f, 1.0f};
-            const float pt0[2] = {p0.x, p0.y}, pt1[2] = {p1.x, p1.y}, pt2[2] = {p2.x, p2.y};
-            const float nms[2] = {nm.x, nm.y};
-#pragma unroll
-            for (int s = 0; s < 2; s++) {
-                a0[s] = a0[s] + h * pt0[s];
-                a1[s] = a1[s] + h * pt1[s];
-                a2[s] = a2[s] + h * pt2[s];
-                const float len = sqrtf(a0[s] * a0[s] + a1[s] * a1[s] + a2[s] * a2[s]);
-                float scale = (nms[s] == 0.0f) ? 0.0f : 1.0f;
-                scale = (len != 0.0f) ? scale / len : 0.0f;
-                a0[s] *= scale;
-                a1[s] *= scale;
-                a2[s] *= scale;
-            }
-            if (ok) {
-                st_off<float2>(mpred, o, float2{a0[0], a0[1]});
-                st_off<float2>(mpred + N, o, float2{a1[0], a1[1]});
-                st_off<float2>(mpred + 2 * N, o, float2{a2[0], a2[1]});
-            }
-        }
-        if (msat_map) {
-            const float2 ms = ld_off<float2>(msat_map, o);
-            a0[0] *= ms.x; a1[0] *= ms.x; a2[0] *= ms.x;
-            a0[1] *= ms.y; a1[1] *= ms.y; a2[1] *= ms.y;
-        }
-        v[0][q] = float2{a0[0], a0[1]};
-        v[1][q] = float2{a1[0], a1[1]};
-        v[2][q] = float2{a2[0], a2[1]};
-    }
-    __syncthreads();
-    float2 u[3][C::G][C::T];
-#pragma unroll
-    for (int c = 0; c < 3; c++) row_forward<LX, !FULL>(v[c], u[c], r, t, sbuf, stw);
-
-    const int lane_row0 = (threadIdx.x & 63) - t;
-    const unsigned ob = (unsigned)((size_t)crow * g.sx * sizeof(float2));
-#pragma unroll
-    for (int c = 0; c < 3; c++) {
-        float2 *dst = xb + (size_t)c * g.nrows * g.sx;
-        static_for<C::G * C::T>([&](auto I) {
-            constexpr int gg = decltype(I)::value / C::T, k2 = decltype(I)::value % C::T;
-            const float2 b = mirror_conj<LX, gg, k2>(u[c], t, lane_row0);
-            const float2 a = u[c][gg][k2];
-            const int k = t * C::G + gg + C::E * k2;
-            const float2 Ev = float2{0.5f * (a.x + b.x), 0.5f * (a.y + b.y)};
-            const float2 Ov = float2{0.5f * (a.y - b.y), -0.5f * (a.x - b.x)};
-            const float2 X = cadd(Ev, cmul(Ov, srtw[k]));
-            if (ok) st_off<float2>(dst, ob + (unsigned)k * 8u, X);
-            if (ok && k == 0) {
-                // Nyquist bin: E(0) - O(0) = Re Z0 - Im Z0
-                st_off<float2>(dst, ob + (unsigned)LX * 8u, float2{a.x - a.y, 0.0f});
-            }
-        });
-    }
-}
-
-// Two neighbouring cells (k0 even, k0 + 1 < n2) of one row at once, for
-// uniform material parameters (no maps) and no spin-transfer term: the same
-// per-cell arithmetic as cell_epilogue (exchange6 order dim 0, dim 2, dim 1;
-// SumUpdater order), with 8-byte loads and stores of the cell pair.
-
-__device__ __forceinline__ int nb_lo(int pos, int len, int wrap)
-{
-    int lo = pos - 1;
-    lo = (lo < 0 && wrap) ? len + lo : lo;
-    return max(lo, 0);
-}
-__device__ __forceinline__ int nb_hi(int pos, int len, int wrap)
-{
-    int hi = pos + 1;
-    hi = (hi == len && wrap) ? hi - len : hi;
-    return min(hi, len - 1);
-}
-
-template <int MODE>
-__device__ __forceinline__ void pair_epilogue(const DemagGeom &g, const FieldParams &fp,
-                                              const EpilogueIO &io, unsigned I0, int i, int j,
-                                              int k0, const float hd0[3], const float hd1[3],
-                                              float &errmax)
-{
-    const size_t N = g.N;
-    const unsigned b0 = I0 * 4u; // byte offset of cell I0 in a component array
-    if (MODE == EPI_B) {
-#pragma unroll
-        for (int c = 0; c < 3; c++) st_off<float2>(io.hout + c * N, b0, float2{hd0[c], hd1[c]});
-        return;
-    }
-    const float *__restrict__ m = io.m;
-    float m0[3], m1[3], H0[3], H1[3];
-#pragma unroll
-    for (int c = 0; c < 3; c++) {
-        const float2 a = ld_off<float2>(m + c * N, b0);
-        m0[c] = a.x;
-        m1[c] = a.y;
-    }
-    // neighbour byte offsets: dim 0 (planes), dim 2 (along the row), dim 1
-    // (rows); negative steps wrap in the unsigned sum and land on valid cells
-    const int s0 = g.n1 * g.n2, s1 = g.n2;
-    const unsigned o0l = b0 + 4u * (unsigned)((nb_lo(i, g.n0, fp.pbc[0]) - i) * s0);
-    const unsigned o0h = b0 + 4u * (unsigned)((nb_hi(i, g.n0, fp.pbc[0]) - i) * s0);
-    const unsigned o1l = b0 + 4u * (unsigned)((nb_lo(j, g.n1, fp.pbc[1]) - j) * s1);
-    const unsigned o1h = b0 + 4u * (unsigned)((nb_hi(j, g.n1, fp.pbc[1]) - j) * s1);
-    const unsigned o2l = b0 + 4u * (unsigned)(nb_lo(k0, g.n2, fp.pbc[2]) - k0); // left of cell 0
-    const unsigned o2h = b0 + 4u * (unsigned)(nb_hi(k0 + 1, g.n2, fp.pbc[2]) - k0); // right of 1
-    const float p0 = fp.ex_pre * fp.ex_c[0], p1 = fp.ex_pre * fp.ex_c[1],
-                p2 = fp.ex_pre * fp.ex_c[2];
-#pragma unroll
-    for (int c = 0; c < 3; c++) {
-        const float *mc = m + c * N;
-        const float2 zl = ld_off<float2>(mc, o0l), zh = ld_off<float2>(mc, o0h);
-        const float2 yl = ld_off<float2>(mc, o1l), yh = ld_off<float2>(mc, o1h);
-        const float xl = ld_off<float>(mc, o2l), xh = ld_off<float>(mc, o2h);
-        float e0 = p0 * ((zl.x - m0[c]) + (zh.x - m0[c]));
-        float e1 = p0 * ((zl.y - m1[c]) + (zh.y - m1[c]));
-        e0 += p2 * ((xl - m0[c]) + (m1[c] - m0[c]));
-        e1 += p2 * ((m0[c] - m1[c]) + (xh - m1[c]));
-        e0 += p1 * ((yl.x - m0[c]) + (yh.x - m0[c]));
-        e1 += p1 * ((yl.y - m1[c]) + (yh.y - m1[c]));
-        H0[c] = hd0[c] + e0 + fp.hext[c];
-        H1[c] = hd1[c] + e1 + fp.hext[c];
-    }
-    if (fp.anis) {
-        const float k = fp.ku2;
-        const float mu0 = m0[0] * fp.u[0] + m0[1] * fp.u[1] + m0[2] * fp.u[2];
-        const float mu1 = m1[0] * fp.u[0] + m1[1] * fp.u[1] + m1[2] * fp.u[2];
-#pragma unroll
-        for (int c = 0; c < 3; c++) {
-            H0[c] += k * mu0 * fp.u[c];
-            H1[c] += k * mu1 * fp.u[c];
-        }
-    }
-    if (io.hout) {
-#pragma unroll
-        for (int c = 0; c < 3; c++) st_off<float2>(io.hout + c * N, b0, float2{H0[c], H1[c]});
-    }
-    if (MODE == EPI_HEFF && io.tout == nullptr) return;
-    float t0[3], t1[3];
-    llg_torque(m0, H0, fp.alpha, t0);
-    llg_torque(m1, H1, fp.alpha, t1);
-    if (io.tout) {
-#pragma unroll
-        for (int c = 0; c < 3; c++) st_off<float2>(io.tout + c * N, b0, float2{t0[c], t1[c]});
-    }
-    if (MODE == EPI_CORR) {
-        const float hh = 0.5f * io.h, mh = -0.5f * io.h;
-        float a0[3], a1[3];
-        float d = 0.0f;
-#pragma unroll
-        for (int c = 0; c < 3; c++) {
-            const float2 p = ld_off<float2>(io.tau0 + c * N, b0);
-            a0[c] = m0[c] + (hh * t0[c] + mh * p.x);
-            a1[c] = m1[c] + (hh * t1[c] + mh * p.y);
-            d = fmaxf(d, fmaxf(fabsf(t0[c] - p.x), fabsf(t1[c] - p.y)));
-        }
-        errmax = fmaxf(errmax, (d != d) ? __builtin_inff() : d);
-        const float l0 = sqrtf(a0[0] * a0[0] + a0[1] * a0[1] + a0[2] * a0[2]);
-        const float l1 = sqrtf(a1[0] * a1[0] + a1[1] * a1[1] + a1[2] * a1[2]);
-        const float q0 = (l0 != 0.0f) ? 1.0f / l0 : 0.0f;
-        const float q1 = (l1 != 0.0f) ? 1.0f / l1 : 0.0f;
-#pragma unroll
-        for (int c = 0; c < 3; c++)
-            st_off<float2>(io.mnew + c * N, b0, float2{a0[c] * q0, a1[c] * q1});
-    }
-}
-
-// xi: C2R along x (imaginary parts of DC and Nyquist ignored), keep k < n2,
-// then the fused field / torque / RK epilogue
-template <int LX, int MODE, bool FULL>
-__global__ void __launch_bounds__(RowTile<LX>::NT)
-xi_fast(DemagGeom g, const float2 *__restrict__ tw, const float2 *__restrict__ rtw,
-        const float2 *__restrict__ xb, FieldParams fp, EpilogueIO io)
-{
-    using C = RowTile<LX>;
-    constexpr int NQ = FULL ? C::E : C::E / 2; // cell pairs 2 (t + T q) < n2
-    extern __shared__ float2 sm[];
-    float2 *sbuf = sm, *stw = sm + C::E * C::S, *srtw = stw + LX;
-    const int tile = xcd_tile(blockIdx.x, gridDim.x);
-    const int r = threadIdx.x / C::T, t = threadIdx.x % C::T;
-    const int row = tile * C::R + r;
-    const bool ok = row < g.nrows;
-    const int crow = ok ? row : (int)g.nrows - 1;
-    for (int q = threadIdx.x; q < LX; q += C::NT) stw[q] = tw[q];
-    for (int q = threadIdx.x; q <= LX; q += C::NT) srtw[q] = rtw[q];
-    __syncthreads();
-    const int lane_row0 = (threadIdx.x & 63) - t;
-    const unsigned ob = (unsigned)((size_t)crow * g.sx * sizeof(float2));
-    float2 v[3][C::E];
-#pragma unroll
-    for (int c = 0; c < 3; c++) {
-        const float2 *src = xb + (size_t)c * g.nrows * g.sx;
-        float2 u[C::G][C::T];
-#pragma unroll
-        for (int gg = 0; gg < C::G; gg++)
-#pragma unroll
-            for (int k2 = 0; k2 < C::T; k2++) {
-                const int k = t * C::G + gg + C::E * k2;
-                float2 x = ld_off<float2>(src, ob + (unsigned)k * 8u);
-                if (k == 0) x.y = 0.0f;
-                u[gg][k2] = x;
-            }
-        const float2 nyq = float2{ld_off<float2>(src, ob + (unsigned)LX * 8u).x, 0.0f};
-        float2 z[C::G][C::T];
-        static_for<C::G * C::T>([&](auto I) {
-            constexpr int gg = decltype(I)::value / C::T, k2 = decltype(I)::value % C::T;
-            float2 b = mirror_conj<LX, gg, k2>(u, t, lane_row0);
-            const int k = t * C::G + gg + C::E * k2;
-            if (k == 0) b = nyq; // conj(X[LX]) with the imaginary part dropped
-            const float2 a = u[gg][k2];
-            const float2 Ev = cadd(a, b);
-            const float2 Ov = cmulc(csub(a, b), srtw[k]);
-            z[gg][k2] = float2{Ev.x - Ov.y, Ev.y + Ov.x};
-        });
-        row_inverse<LX>(z, v[c], r, t, sbuf, stw);
-    }
-
-    float errmax = 0.0f;
-    // this kernel serves uniform parameters without the spin-transfer term
-    // (fast_xi_applies); cell pairs with 8-byte I/O (n2 is even on this path:
-    // n2 = LX padded to 2 LX, or periodic with n2 = 2 LX)
-    if (ok) {
-        const int i = row / g.n1, j = row - i * g.n1;
-        // the pairs' loads are left to the scheduler: a scheduling barrier per
-        // pair (one pair's loads in flight at a time) measured 3 % slower
-#pragma unroll
-        for (int q = 0; q < NQ; q++) {
-            const int k0 = 2 * (t + C::T * q);
-            float hd0[3], hd1[3];
-#pragma unroll
-            for (int c = 0; c < 3; c++) {
-                hd0[c] = fp.demag_scale * v[c][q].x;
-                hd1[c] = fp.demag_scale * v[c][q].y;
-            }
-            pair_epilogue<MODE>(g, fp, io, (unsigned)(row * g.n2 + k0), i, j, k0, hd0, hd1,
-                                errmax);
-        }
-    }
-    if (MODE == EPI_CORR) block_max_to(errmax, io.err);
-}
-
 // ------------------------------------------------------------ host side
 
-template <typename K> static void allow_lds_fast(K kernel, size_t bytes)
-{
-    if (bytes > 65536)
-        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(kernel),
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
-}
-
-#define MMX_FAST_LENGTHS(X) X(8) X(16) X(32) X(64) X(128) X(256) X(512) X(1024) X(2048)
 
 int fast_yf(const DemagPlanImpl *P, hipStream_t st)
 {
@@ -889,43 +163,17 @@ int fast_xf(const DemagPlanImpl *P, int pred, const float *m, const float *msat_
     return MMX_OK;
 }
 
-template <int L, int MODE>
-static void launch_xi(const DemagPlanImpl *P, const FieldParams &fp, const EpilogueIO &io,
-                      hipStream_t st)
-{
-    using C = RowTile<L>;
-    const int nb = div_up(P->g.nrows, C::R);
-    auto kern = (P->g.n2 == 2 * L) ? xi_fast<L, MODE, true> : xi_fast<L, MODE, false>;
-    allow_lds_fast(kern, C::LDS);
-    kern<<<dim3(nb), dim3(C::NT), C::LDS, st>>>(P->g, P->fx.tw, P->fx.rtw, P->xbuf, fp, io);
-}
-
 bool fast_xi_applies(const FieldParams &fp)
 {
-    return !fp.msat_map && !fp.aex_map && !fp.alpha_map && !fp.ku_map && !fp.zl && !fp.htherm;
+    return !fp.msat_map && !fp.aex_map && !fp.alpha_map && !fp.ku_map && !fp.zl && !fp.htherm &&
+           !fp.bext_map && !fp.u_map;
 }
 
 int fast_xi(const DemagPlanImpl *P, int mode, const FieldParams &fp, const EpilogueIO &io,
             hipStream_t st)
 {
-    const DemagGeom &g = P->g;
-    MMX_REQUIRE(fast_x_addressable(g), MMX_ERR_SIZE, "fast_xi: unsupported geometry");
-    switch (g.Lx) {
-#define CASE(L)                                                                              \
-    case L:                                                                                  \
-        switch (mode) {                                                                      \
-        case EPI_B: launch_xi<L, EPI_B>(P, fp, io, st); break;                               \
-        case EPI_HEFF: launch_xi<L, EPI_HEFF>(P, fp, io, st); break;                         \
-        case EPI_TAU0: launch_xi<L, EPI_TAU0>(P, fp, io, st); break;                         \
-        default: launch_xi<L, EPI_CORR>(P, fp, io, st); break;                               \
-        }                                                                                    \
-        break;
-        MMX_FAST_LENGTHS(CASE)
-#undef CASE
-    default: set_error("fast_xi: length %d", g.Lx); return MMX_ERR_SIZE;
-    }
-    MMX_CHECK_LAUNCH();
-    return MMX_OK;
+    if (mode != EPI_B && !fast_xi_applies(fp)) return fast_xi_general(P, mode, fp, io, st);
+    return fast_xi_modes<false>(P, mode, fp, io, st);
 }
 
 } // namespace mmx

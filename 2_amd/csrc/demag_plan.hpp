@@ -48,6 +48,12 @@ struct FieldParams {
     float zl_step[3];           // 1/(12 cell)
     float inv_gamma;            // weight of zzt in the torque sum
     const float *htherm = nullptr; // temperature/brown field (3 N) or NULL
+    // MASK quantities (NULL = uniform): B_ext (3 N, hext_c * map_c), anisU
+    // (3 N, u_c * map_c), polarisation and xi (N each, Zhang-Li prefactor)
+    const float *bext_map = nullptr;
+    const float *u_map = nullptr;
+    const float *pol_map = nullptr;
+    const float *xi_map = nullptr;
 };
 
 // Per-launch state of the epilogue (pointers that change between calls).
@@ -130,8 +136,12 @@ int fast_yi(const DemagPlanImpl *P, hipStream_t st);
 int fast_y2d(const DemagPlanImpl *P, hipStream_t st);
 int fast_xi(const DemagPlanImpl *P, int mode, const FieldParams &fp, const EpilogueIO &io,
             hipStream_t st);
-// the fast C2R pass covers uniform parameters without the spin-transfer term
+// the fast C2R pass's pair epilogue covers uniform parameters without the
+// spin-transfer term; otherwise fast_xi runs the general per-cell epilogue
+// (fast_xi_general, xi_gen.hip)
 bool fast_xi_applies(const FieldParams &fp);
+int fast_xi_general(const DemagPlanImpl *P, int mode, const FieldParams &fp, const EpilogueIO &io,
+                    hipStream_t st);
 
 } // namespace mmx
 

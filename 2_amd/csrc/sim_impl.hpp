@@ -20,6 +20,8 @@ struct mmx_sim {
     double j[3] = {0, 0, 0}, pol = 1.0, xi = 0.05; // mod_zhang-li.go:31-36
     float *msat_map = nullptr, *aex_map = nullptr, *alpha_map = nullptr, *ku_map = nullptr,
           *j_map = nullptr;
+    // MASK quantities B_ext, anisU (3 components), polarisation, xi
+    float *bext_map = nullptr, *u_map = nullptr, *pol_map = nullptr, *xi_map = nullptr;
 
     // solver/rk12 (solve_rk12.go:33-72 + mod_micromag.go:31-33)
     double t = 0, dt = 1e-15, mindt = 1e-15, maxdt = 1e38, maxerr = 1.0 / 2000.0;

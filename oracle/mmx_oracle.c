@@ -608,7 +608,7 @@ void orc_exchange6(const int n[3], const int pbc[3], const double cell[3],
 
 void orc_uniaxial(long N, double ku2, const double *ku_map,
                   const double *msat_map, const double u[3],
-                  const double *m, double *h)
+                  const double *u_map, const double *m, double *h)
 {
 #pragma omp parallel for schedule(static)
     for (long i = 0; i < N; i++) {
@@ -618,10 +618,13 @@ void orc_uniaxial(long N, double ku2, const double *ku_map,
             if (ms == 0.0) ms = 1.0; /* uniaxialanisotropy.cu:30-40 */
         }
         double k = ku_map ? (ku2 / ms) * ku_map[i] : ku2 / ms;
-        double mu = m[i] * u[0] + m[N + i] * u[1] + m[2 * N + i] * u[2];
-        h[i] = k * mu * u[0];
-        h[N + i] = k * mu * u[1];
-        h[2 * N + i] = k * mu * u[2];
+        double uc[3];
+        for (int c = 0; c < 3; c++) /* mul * map, uniaxialanisotropy.cu:51-79 */
+            uc[c] = u_map ? u[c] * u_map[c * N + i] : u[c];
+        double mu = m[i] * uc[0] + m[N + i] * uc[1] + m[2 * N + i] * uc[2];
+        h[i] = k * mu * uc[0];
+        h[N + i] = k * mu * uc[1];
+        h[2 * N + i] = k * mu * uc[2];
     }
 }
 
@@ -779,13 +782,14 @@ int orc_heff(const orc_problem *P, const double *m, double *h,
     }
     if (P->anis) {
         double ku2 = 2.0 * P->ku / (ORC_MU0 * P->msat);
-        orc_uniaxial(N, ku2, P->ku_map, P->msat_map, P->anis_u, m, tmp);
+        orc_uniaxial(N, ku2, P->ku_map, P->msat_map, P->anis_u, P->u_map, m, tmp);
         for (long t = 0; t < 3 * N; t++) h[t] += tmp[t];
         if (h_anis) memcpy(h_anis, tmp, sizeof(double) * 3 * N);
     }
-    for (int c = 0; c < 3; c++) {
+    for (int c = 0; c < 3; c++) { /* B_ext mask: vecMadd, mul * map (add.cu:66-101) */
         double add = P->bext[c] / ORC_MU0;
-        for (long t = 0; t < N; t++) h[c * N + t] += add;
+        for (long t = 0; t < N; t++)
+            h[c * N + t] += P->bext_map ? add * P->bext_map[c * N + t] : add;
     }
     free(tmp);
     return 0;
@@ -802,7 +806,7 @@ int orc_torque_full(const orc_problem *P, const double *m, const double *h,
         /* pred = P muB / (e Ms), mod_zhang-li.go:62-68 */
         double pre = P->pol * ORC_MUB / (ORC_QE * P->msat);
         orc_zhangli(P->n, P->pbc, P->cell, m, P->jmask, P->jmul, P->msat_map,
-                    NULL, NULL, P->alpha_map, pre, P->xi, P->alpha, stt);
+                    P->pol_map, P->xi_map, P->alpha_map, pre, P->xi, P->alpha, stt);
         for (long t = 0; t < 3 * N; t++) tau[t] += stt[t] / P->gamma;
         free(stt);
     }

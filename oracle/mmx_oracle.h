@@ -96,10 +96,11 @@ void orc_exchange6(const int n[3], const int pbc[3], const double cell[3],
                    double pre, const double *msat_map, const double *aex_map,
                    const double *m, double *h);
 
-/* libmumax2/uniaxialanisotropy.cu:12-121; ku2 = 2 Ku/(mu0 Ms). */
+/* libmumax2/uniaxialanisotropy.cu:12-121; ku2 = 2 Ku/(mu0 Ms); u_map: the
+ * anisU mask (3 components, u_c = u[c] * u_map_c) or NULL. */
 void orc_uniaxial(long N, double ku2, const double *ku_map,
                   const double *msat_map, const double u[3],
-                  const double *m, double *h);
+                  const double *u_map, const double *m, double *h);
 
 /* libmumax2/torque.cu:13-78: tau/gamma, internal-coordinate formula. */
 void orc_torque(long N, const double *m, const double *h,
@@ -129,6 +130,11 @@ typedef struct {
     int demag;  int p[3];   const double *khat6;
     int anis;   double ku;  const double *ku_map;  double anis_u[3];
     int zhangli; double jmul[3]; const double *jmask; double pol; double xi;
+    /* MASK quantities (NULL = uniform): B_ext (3 components,
+     * mod_zeeman.go:26), anisU (3, mod_uniaxialanisotropy.go:30),
+     * polarisation and xi (zhang-li_torque.cu:167,171) */
+    const double *bext_map; const double *u_map;
+    const double *pol_map;  const double *xi_map;
 } orc_problem;
 
 /* SumUpdater for H_eff (engine/update_sum.go:29-75, modules/hfield.go:19-29):

@@ -43,7 +43,7 @@ def _load():
     lib.orc_demag_brute.argtypes = [_ip, _ip, _fp, _dp, _dp, _dp]
     lib.orc_exchange6.argtypes = [_ip, _ip, _dp, C.c_double, _dp, _dp, _dp, _dp]
     lib.orc_exchange6.restype = None
-    lib.orc_uniaxial.argtypes = [C.c_long, C.c_double, _dp, _dp, _dp, _dp, _dp]
+    lib.orc_uniaxial.argtypes = [C.c_long, C.c_double, _dp, _dp, _dp, _dp, _dp, _dp]
     lib.orc_uniaxial.restype = None
     lib.orc_torque.argtypes = [C.c_long, _dp, _dp, _dp, C.c_double, _dp]
     lib.orc_torque.restype = None
@@ -192,11 +192,12 @@ def exchange6(n, pbc, cell, pre, m, msat_map=None, aex_map=None):
     return h
 
 
-def uniaxial(m, ku2, u, ku_map=None, msat_map=None):
+def uniaxial(m, ku2, u, ku_map=None, msat_map=None, u_map=None):
     m = _f64(m)
     N = m.size // 3
     h = np.zeros_like(m)
-    lib().orc_uniaxial(N, float(ku2), _ptr(_f64(ku_map)), _ptr(_f64(msat_map)), _d3(u), _ptr(m), _ptr(h))
+    lib().orc_uniaxial(N, float(ku2), _ptr(_f64(ku_map)), _ptr(_f64(msat_map)), _d3(u),
+                       _ptr(_f64(u_map)), _ptr(m), _ptr(h))
     return h
 
 
@@ -236,6 +237,7 @@ class _Problem(C.Structure):
         ("anis", C.c_int), ("ku", C.c_double), ("ku_map", _dp), ("anis_u", C.c_double * 3),
         ("zhangli", C.c_int), ("jmul", C.c_double * 3), ("jmask", _dp), ("pol", C.c_double),
         ("xi", C.c_double),
+        ("bext_map", _dp), ("u_map", _dp), ("pol_map", _dp), ("xi_map", _dp),
     ]
 
 
@@ -258,7 +260,8 @@ class Problem:
     def __init__(self, n, cell, pbc=(0, 0, 0), msat=8e5, aex=1.3e-11, alpha=0.02,
                  gamma=GAMMA0, bext=(0, 0, 0), demag=True, ku=None, anis_u=(1, 0, 0),
                  j=None, pol=1.0, xi=0.05, msat_map=None, aex_map=None, alpha_map=None,
-                 ku_map=None, jmask=None, kernel=None, accuracy=6):
+                 ku_map=None, jmask=None, kernel=None, accuracy=6, bext_map=None,
+                 anis_u_map=None, pol_map=None, xi_map=None):
         self.n = tuple(int(x) for x in n)
         self.N = int(np.prod(self.n))
         self.cell = tuple(float(c) for c in cell)
@@ -279,6 +282,10 @@ class Problem:
         P.alpha_map = self._map("alpha_map", alpha_map)
         P.ku_map = self._map("ku_map", ku_map)
         P.jmask = self._map("jmask", jmask)
+        P.bext_map = self._map("bext_map", bext_map)
+        P.u_map = self._map("u_map", anis_u_map)
+        P.pol_map = self._map("pol_map", pol_map)
+        P.xi_map = self._map("xi_map", xi_map)
         P.demag = 1 if demag else 0
         P.p = _i3(self.p)
         if demag:

@@ -241,6 +241,48 @@ def test_uniaxial_closed_form():
     assert np.allclose(h2, h / msd[None], rtol=1e-14)
 
 
+def test_mask_quantities_in_the_problem():
+    """The MASK paths of B_ext, anisU, polarisation and xi: B_ext's map scales
+    its uniform term cell by cell (vecMadd mul * map); anisU's map gives each
+    cell its own axis u_c = mul_c * map_c (uniaxialanisotropy.cu:51-79);
+    polarisation and xi enter the Zhang-Li term only through the cell's own
+    prefactor, so a map equals, cell by cell, the uniform value it holds there."""
+    rng = np.random.default_rng(5)
+    n, cell = (2, 5, 6), (3e-9, 4e-9, 5e-9)
+    m = O.normalize(rng.normal(size=(3,) + n))
+    bext = (0.01, -0.02, 0.03)
+    bmap = rng.uniform(-1, 2, size=(3,) + n)
+    base = O.Problem(n, cell, demag=False)
+    with_b = O.Problem(n, cell, demag=False, bext=bext, bext_map=bmap)
+    want = np.array(bext)[:, None, None, None] / O.MU0 * bmap
+    assert np.allclose(with_b.heff(m) - base.heff(m), want, rtol=1e-12, atol=1e-9)
+    # anisU map against the closed form with each cell's axis
+    umul = np.array([0.5, 2.0, -1.0])
+    umap = rng.normal(size=(3,) + n)
+    h = O.uniaxial(m, 2.5, umul, u_map=umap)
+    uc = umul[:, None, None, None] * umap
+    mu = (m * uc).sum(axis=0)
+    assert np.allclose(h, 2.5 * mu[None] * uc, rtol=1e-13)
+    pa = O.Problem(n, cell, demag=False, ku=5e5, anis_u=tuple(umul), anis_u_map=umap)
+    pb = O.Problem(n, cell, demag=False, ku=5e5, anis_u=tuple(umul))
+    ku2 = 2 * 5e5 / (O.MU0 * 8e5)
+    da = pa.heff(m) - pb.heff(m)
+    assert np.allclose(da, O.uniaxial(m, ku2, umul, u_map=umap) - O.uniaxial(m, ku2, umul),
+                       rtol=1e-12, atol=1e-6)
+    # polarisation / xi maps: cell by cell the uniform value
+    j = (0.0, 3e11, 1e12)
+    pol_map = rng.choice([0.3, 0.9], size=n)
+    xi_map = rng.choice([0.5, 2.0], size=n)
+    tau_map = O.Problem(n, cell, demag=False, j=j, pol=0.56, xi=0.05, pol_map=pol_map,
+                        xi_map=xi_map).torque(m, base.heff(m))
+    for pv in (0.3, 0.9):
+        for xv in (0.5, 2.0):
+            t = O.Problem(n, cell, demag=False, j=j, pol=0.56 * pv,
+                          xi=0.05 * xv).torque(m, base.heff(m))
+            sel = (pol_map == pv) & (xi_map == xv)
+            assert np.allclose(tau_map[:, sel], t[:, sel], rtol=1e-12, atol=1e-12)
+
+
 def test_torque_physics():
     rng = np.random.default_rng(1)
     m = O.normalize(rng.normal(size=(3, 4, 4, 4)))
