@@ -127,6 +127,7 @@ template <typename K> static void allow_lds(K kernel, size_t bytes)
 }
 
 bool fast_length(int L) { return fast_length_impl(L); }
+bool fast_ylength(int L) { return fast_ylength_impl(L); }
 
 static int y_and_z_passes(DemagPlanImpl *P, hipStream_t st)
 {

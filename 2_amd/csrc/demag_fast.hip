@@ -1,12 +1,21 @@
-// demag_fast.hip -- host launchers of the demag passes for power-of-two
-// transform lengths (the kernels are in fast_kernels.hpp; the general xi
-// epilogue is instantiated in xi_gen.hip).
-#include "fast_kernels.hpp"
+// demag_fast.hip -- dispatch of the register-FFT demag passes on the plan's
+// transform lengths, and the launchers of the power-of-two lengths (the
+// kernels are in fast_kernels.hpp; the general xi epilogue of these lengths is
+// instantiated in xi_gen.hip, lengths with a factor 3, 5 or 7 in fast_r3.hip,
+// fast_r5.hip and fast_r7.hip).
+#include "fast_launch.hpp"
 
 namespace mmx {
 
-// ------------------------------------------------------------ host side
+MMX_POW2_LENGTHS(MMX_INST_XF)
+MMX_POW2_LENGTHS(MMX_INST_Y)
+MMX_POW2_LENGTHS(MMX_INST_Z)
+MMX_POW2_LENGTHS(MMX_INST_Y2D)
+#define MMX_INST_XI_PLAIN(L) MMX_INST_XI(L, false)
+MMX_POW2_LENGTHS(MMX_INST_XI_PLAIN)
+#undef MMX_INST_XI_PLAIN
 
+// ------------------------------------------------------------ host side
 
 int fast_yf(const DemagPlanImpl *P, hipStream_t st)
 {
@@ -14,21 +23,11 @@ int fast_yf(const DemagPlanImpl *P, hipStream_t st)
     MMX_REQUIRE(g.n1 == g.p1 || 2 * g.n1 == g.p1, MMX_ERR_SIZE, "fast_yf: rows %d of %d", g.n1, g.p1);
     switch (g.p1) {
 #define CASE(L)                                                                              \
-    case L: {                                                                                \
-        using C = ColTile<L>;                                                                \
-        const int ntx = div_up(g.h2, C::W);                                                  \
-        auto kern = (g.n1 == g.p1) ? yf_fast<L, true> : yf_fast<L, false>;                   \
-        allow_lds_fast(kern, C::LDS);                                                        \
-        kern<<<dim3(ntx * g.n0 * 3), dim3(C::NT), C::LDS, st>>>(g, ntx, P->fy.tw,            \
-                                                                      P->xbuf, P->ybuf);     \
-        break;                                                                               \
-    }
-        MMX_FAST_LENGTHS(CASE)
+    case L: return yf_len<L>(P, st);
+        MMX_ALL_LENGTHS(CASE)
 #undef CASE
     default: set_error("fast_yf: length %d", g.p1); return MMX_ERR_SIZE;
     }
-    MMX_CHECK_LAUNCH();
-    return MMX_OK;
 }
 
 int fast_yi(const DemagPlanImpl *P, hipStream_t st)
@@ -37,21 +36,11 @@ int fast_yi(const DemagPlanImpl *P, hipStream_t st)
     MMX_REQUIRE(g.n1 == g.p1 || 2 * g.n1 == g.p1, MMX_ERR_SIZE, "fast_yi: rows %d of %d", g.n1, g.p1);
     switch (g.p1) {
 #define CASE(L)                                                                              \
-    case L: {                                                                                \
-        using C = ColTile<L>;                                                                \
-        const int ntx = div_up(g.h2, C::W);                                                  \
-        auto kern = (g.n1 == g.p1) ? yi_fast<L, true> : yi_fast<L, false>;                   \
-        allow_lds_fast(kern, C::LDS);                                                        \
-        kern<<<dim3(ntx * g.n0 * 3), dim3(C::NT), C::LDS, st>>>(g, ntx, P->fy.tw,            \
-                                                                      P->ybuf, P->xbuf);     \
-        break;                                                                               \
-    }
-        MMX_FAST_LENGTHS(CASE)
+    case L: return yi_len<L>(P, st);
+        MMX_ALL_LENGTHS(CASE)
 #undef CASE
     default: set_error("fast_yi: length %d", g.p1); return MMX_ERR_SIZE;
     }
-    MMX_CHECK_LAUNCH();
-    return MMX_OK;
 }
 
 // The z pass addresses the spectrum and the kernel octant with 32-bit byte
@@ -63,7 +52,7 @@ bool fast_z_addressable(const DemagGeom &g)
     switch (g.p0) {
 #define CASE(L)                                                                              \
     case L: T = ZTile<L>::T; break;
-        MMX_FAST_LENGTHS(CASE)
+        MMX_SHORT_LENGTHS(CASE)
 #undef CASE
     default: return false;
     }
@@ -77,24 +66,13 @@ int fast_z(const DemagPlanImpl *P, hipStream_t st)
 {
     const DemagGeom &g = P->g;
     MMX_REQUIRE(fast_z_addressable(g), MMX_ERR_SIZE, "fast_z: unsupported geometry");
-    const bool full = g.n0 == g.p0;
     switch (g.p0) {
 #define CASE(L)                                                                              \
-    case L: {                                                                                \
-        using Z = ZTile<L>;                                                                  \
-        const int ntx = div_up(g.h2, Z::W);                                                  \
-        auto kern = full ? z_fast<L, true> : z_fast<L, false>;                               \
-        allow_lds_fast(kern, Z::LDS);                                                        \
-        kern<<<dim3(ntx * g.p1), dim3(Z::NT), Z::LDS, st>>>(g, ntx, P->fz.tw, P->kern,       \
-                                                            P->ybuf);                        \
-        break;                                                                               \
-    }
-        MMX_FAST_LENGTHS(CASE)
+    case L: return z_len<L>(P, st);
+        MMX_SHORT_LENGTHS(CASE)
 #undef CASE
     default: set_error("fast_z: length %d", g.p0); return MMX_ERR_SIZE;
     }
-    MMX_CHECK_LAUNCH();
-    return MMX_OK;
 }
 
 int fast_y2d(const DemagPlanImpl *P, hipStream_t st)
@@ -103,21 +81,11 @@ int fast_y2d(const DemagPlanImpl *P, hipStream_t st)
     MMX_REQUIRE(g.n1 == g.p1 || 2 * g.n1 == g.p1, MMX_ERR_SIZE, "fast_y2d: rows %d of %d", g.n1, g.p1);
     switch (g.p1) {
 #define CASE(L)                                                                              \
-    case L: {                                                                                \
-        using C = ColTile<L>;                                                                \
-        const int ntx = div_up(g.h2, C::W);                                                  \
-        auto kern = (g.n1 == g.p1) ? y2d_fast<L, true> : y2d_fast<L, false>;                 \
-        allow_lds_fast(kern, C::LDS);                                                        \
-        kern<<<dim3(ntx), dim3(C::NT), C::LDS, st>>>(g, ntx, P->fy.tw, P->kern,              \
-                                                            P->xbuf);                        \
-        break;                                                                               \
-    }
-        MMX_FAST_LENGTHS(CASE)
+    case L: return y2d_len<L>(P, st);
+        MMX_SHORT_LENGTHS(CASE)
 #undef CASE
     default: set_error("fast_y2d: length %d", g.p1); return MMX_ERR_SIZE;
     }
-    MMX_CHECK_LAUNCH();
-    return MMX_OK;
 }
 
 // The x passes address cells with 32-bit byte offsets from the component
@@ -137,43 +105,32 @@ int fast_xf(const DemagPlanImpl *P, int pred, const float *m, const float *msat_
     MMX_REQUIRE(fast_x_addressable(g), MMX_ERR_SIZE, "fast_xf: unsupported geometry");
     switch (g.Lx) {
 #define CASE(L)                                                                              \
-    case L: {                                                                                \
-        using C = RowTile<L>;                                                                \
-        const int nb = div_up(g.nrows, C::R);                                                \
-        const bool full = g.n2 == 2 * L;                                                     \
-        if (pred) {                                                                          \
-            auto kern = full ? xf_fast<L, 1, true> : xf_fast<L, 1, false>;                   \
-            allow_lds_fast(kern, C::LDS);                                                    \
-            kern<<<dim3(nb), dim3(C::NT), C::LDS, st>>>(                                     \
-                g, P->fx.tw, P->fx.rtw, m, msat_map, tau0, h, norm_map, mpred, P->xbuf);     \
-        } else {                                                                             \
-            auto kern = full ? xf_fast<L, 0, true> : xf_fast<L, 0, false>;                   \
-            allow_lds_fast(kern, C::LDS);                                                    \
-            kern<<<dim3(nb), dim3(C::NT), C::LDS, st>>>(                                     \
-                g, P->fx.tw, P->fx.rtw, m, msat_map, nullptr, 0.0f, nullptr, nullptr,        \
-                P->xbuf);                                                                    \
-        }                                                                                    \
-        break;                                                                               \
-    }
-        MMX_FAST_LENGTHS(CASE)
+    case L: return xf_len<L>(P, pred, m, msat_map, tau0, h, norm_map, mpred, st);
+        MMX_SHORT_LENGTHS(CASE)
 #undef CASE
     default: set_error("fast_xf: length %d", g.Lx); return MMX_ERR_SIZE;
     }
-    MMX_CHECK_LAUNCH();
-    return MMX_OK;
 }
 
 bool fast_xi_applies(const FieldParams &fp)
 {
     return !fp.msat_map && !fp.aex_map && !fp.alpha_map && !fp.ku_map && !fp.zl && !fp.htherm &&
-           !fp.bext_map && !fp.u_map;
+           !fp.bext_map && !fp.u_map && !fp.hdex;
 }
 
 int fast_xi(const DemagPlanImpl *P, int mode, const FieldParams &fp, const EpilogueIO &io,
             hipStream_t st)
 {
-    if (mode != EPI_B && !fast_xi_applies(fp)) return fast_xi_general(P, mode, fp, io, st);
-    return fast_xi_modes<false>(P, mode, fp, io, st);
+    const DemagGeom &g = P->g;
+    MMX_REQUIRE(fast_x_addressable(g), MMX_ERR_SIZE, "fast_xi: unsupported geometry");
+    const bool gen = mode != EPI_B && !fast_xi_applies(fp);
+    switch (g.Lx) {
+#define CASE(L)                                                                              \
+    case L: return gen ? xi_len<L, true>(P, mode, fp, io, st) : xi_len<L, false>(P, mode, fp, io, st);
+        MMX_SHORT_LENGTHS(CASE)
+#undef CASE
+    default: set_error("fast_xi: length %d", g.Lx); return MMX_ERR_SIZE;
+    }
 }
 
 } // namespace mmx

@@ -52,6 +52,9 @@ struct FieldParams {
     // (3 N, u_c * map_c), polarisation and xi (N each, Zhang-Li prefactor)
     const float *bext_map = nullptr;
     const float *u_map = nullptr;
+    // demagexch with an Msat mask: H_dex = conv(m, dex) computed by its own
+    // convolution (3 N), added like H_therm
+    const float *hdex = nullptr;
     const float *pol_map = nullptr;
     const float *xi_map = nullptr;
 };
@@ -125,7 +128,8 @@ int demag_inverse(DemagPlanImpl *P, int mode, const FieldParams &fp,
                   const EpilogueIO &io, hipStream_t st);
 
 // Power-of-two passes (demag_fast.hip)
-bool fast_length(int L);
+bool fast_length(int L);   // x passes, film y, z
+bool fast_ylength(int L);  // 3-D y passes
 int fast_xf(const DemagPlanImpl *P, int pred, const float *m, const float *msat_map,
             const float *tau0, float h, const float *norm_map, float *mpred, hipStream_t st);
 int fast_yf(const DemagPlanImpl *P, hipStream_t st);

@@ -22,7 +22,9 @@ template <int L> struct ColTile {
     static constexpr int G = E / T;
     // length 2048: 2 columns (3 % faster than 4 at 64x2048x2048: 49 KB of LDS
     // per workgroup instead of 83, three workgroups per CU instead of one)
-    static constexpr int W = (L == 2048) ? 2 : (L == 1024 ? 8 : 256 / T);
+    // lengths with an odd factor and more than 32 registers: half as many
+    // columns (LDS below 64 KB)
+    static constexpr int W = (L == 2048) ? 2 : (L == 1024 ? 8 : (E > 32 ? 128 / T : 256 / T));
     static constexpr int S = lds_pitch(T, W, G);
     static constexpr int NT = T * W;                     // threads
     static constexpr int LDS = (E * S + L) * 8;           // transpose + twiddles
@@ -179,7 +181,10 @@ template <int L> struct ZTile {
     static constexpr int E = Split<L>::E;
     static constexpr int G = E / T;
     // 8 columns for L = 128 (measured: 141 us vs 145 us with 16 at 256x256x64)
-    static constexpr int W = (L == 2048) ? 2 : (L == 128 ? 8 : 128 / T);
+    // lengths with an odd factor and more than 16 registers: one wave per
+    // component group (the three groups' LDS below 64 KB)
+    static constexpr int W =
+        (L == 2048) ? 2 : (L == 128 ? 8 : ((!is_pow2(L) && E > 16) ? 64 / T : 128 / T));
     static constexpr int S = lds_pitch(T, W, G);
     static constexpr int NG = T * W;          // threads per component group
     static constexpr int NT = 3 * NG;
@@ -743,44 +748,29 @@ template <typename K> static void allow_lds_fast(K kernel, size_t bytes)
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
 }
 
-#define MMX_FAST_LENGTHS(X) X(8) X(16) X(32) X(64) X(128) X(256) X(512) X(1024) X(2048)
 
-// One xi launch of length L, mode MODE (GEN: the general epilogue).
-template <int L, int MODE, bool GEN>
-static void launch_xi(const DemagPlanImpl *P, const FieldParams &fp, const EpilogueIO &io,
-                      hipStream_t st)
-{
-    using C = RowTile<L>;
-    const int nb = div_up(P->g.nrows, C::R);
-    auto kern = (P->g.n2 == 2 * L) ? xi_fast<L, MODE, true, GEN> : xi_fast<L, MODE, false, GEN>;
-    allow_lds_fast(kern, C::LDS);
-    kern<<<dim3(nb), dim3(C::NT), C::LDS, st>>>(P->g, P->fx.tw, P->fx.rtw, P->xbuf, fp, io);
-}
+// Launchers of one transform length (defined in fast_launch.hpp, instantiated
+// by length in the files that include it).
+template <int L>
+int xf_len(const DemagPlanImpl *P, int pred, const float *m, const float *msat_map,
+           const float *tau0, float h, const float *norm_map, float *mpred, hipStream_t st);
+template <int L> int yf_len(const DemagPlanImpl *P, hipStream_t st);
+template <int L> int yi_len(const DemagPlanImpl *P, hipStream_t st);
+template <int L> int z_len(const DemagPlanImpl *P, hipStream_t st);
+template <int L> int y2d_len(const DemagPlanImpl *P, hipStream_t st);
+template <int L, bool GEN>
+int xi_len(const DemagPlanImpl *P, int mode, const FieldParams &fp, const EpilogueIO &io,
+           hipStream_t st);
 
-// xi over every mode for one GEN choice (demag_fast.hip: GEN false,
-// xi_gen.hip: GEN true)
-template <bool GEN>
-static int fast_xi_modes(const DemagPlanImpl *P, int mode, const FieldParams &fp,
-                         const EpilogueIO &io, hipStream_t st)
-{
-    const DemagGeom &g = P->g;
-    MMX_REQUIRE(fast_x_addressable(g), MMX_ERR_SIZE, "fast_xi: unsupported geometry");
-    switch (g.Lx) {
-#define CASE(L)                                                                              \
-    case L:                                                                                  \
-        switch (mode) {                                                                      \
-        case EPI_B: launch_xi<L, EPI_B, false>(P, fp, io, st); break;                        \
-        case EPI_HEFF: launch_xi<L, EPI_HEFF, GEN>(P, fp, io, st); break;                    \
-        case EPI_TAU0: launch_xi<L, EPI_TAU0, GEN>(P, fp, io, st); break;                    \
-        default: launch_xi<L, EPI_CORR, GEN>(P, fp, io, st); break;                          \
-        }                                                                                    \
-        break;
-        MMX_FAST_LENGTHS(CASE)
-#undef CASE
-    default: set_error("fast_xi: length %d", g.Lx); return MMX_ERR_SIZE;
-    }
-    MMX_CHECK_LAUNCH();
-    return MMX_OK;
-}
+// the lengths of each pass (fast_pow2_length, fast_odd_short, fast_odd_long)
+#define MMX_POW2_LENGTHS(X) X(8) X(16) X(32) X(64) X(128) X(256) X(512) X(1024) X(2048)
+#define MMX_R3_SHORT(X) X(24) X(48) X(96) X(192)
+#define MMX_R5_SHORT(X) X(40) X(80)
+#define MMX_R7_SHORT(X) X(56) X(112)
+#define MMX_R3_LONG(X) X(384)
+#define MMX_R5_LONG(X) X(160) X(320)
+#define MMX_R7_LONG(X) X(224) X(448)
+#define MMX_SHORT_LENGTHS(X) MMX_POW2_LENGTHS(X) MMX_R3_SHORT(X) MMX_R5_SHORT(X) MMX_R7_SHORT(X)
+#define MMX_ALL_LENGTHS(X) MMX_SHORT_LENGTHS(X) MMX_R3_LONG(X) MMX_R5_LONG(X) MMX_R7_LONG(X)
 
 } // namespace mmx

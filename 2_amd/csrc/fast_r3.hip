@@ -1,0 +1,18 @@
+// fast_r3.hip -- the register-FFT passes of the transform lengths with a
+// factor 3 (mixed-radix in-register FFTs, reg_fft.hpp): every pass for the
+// lengths of at most 28 registers per component, the strided 3-D y passes for
+// the longer ones.
+#include "fast_launch.hpp"
+
+namespace mmx {
+
+#define MMX_INST_XI_BOTH(L) MMX_INST_XI(L, false) MMX_INST_XI(L, true)
+MMX_R3_SHORT(MMX_INST_XF)
+MMX_R3_SHORT(MMX_INST_XI_BOTH)
+MMX_R3_SHORT(MMX_INST_Y2D)
+MMX_R3_SHORT(MMX_INST_Z)
+MMX_R3_SHORT(MMX_INST_Y)
+MMX_R3_LONG(MMX_INST_Y)
+#undef MMX_INST_XI_BOTH
+
+} // namespace mmx

@@ -714,7 +714,7 @@ int build_demag_kernel(DemagPlanImpl *P, bool keep_real)
 
 int build_dex_kernel(const DemagPlanImpl *P, int accuracy, double ms, double aex, float *out)
 {
-    MMX_REQUIRE(P && out && accuracy > 0 && ms != 0.0, MMX_ERR_ARG, "build_dex_kernel: bad arguments");
+    MMX_REQUIRE(P && out && accuracy >= 0 && ms != 0.0, MMX_ERR_ARG, "build_dex_kernel: bad arguments");
     DemagPlanImpl Q = *P;   // geometry, cells, periodicity; nothing of P is written
     Q.kern = out;
     Q.real_kernel = nullptr;

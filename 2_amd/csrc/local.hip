@@ -468,7 +468,9 @@ int mmx_exchange6(float *hx, float *hy, float *hz, const float *mx, const float 
     const float c1 = (float)(1.0 / ((double)cell1 * (double)cell1));
     const float c2 = (float)(1.0 / ((double)cell2 * (double)cell2));
     const long nj = div_up(N1, kEJ), nk = div_up(N2, kEK);
-    if (!msat_map && !aex_map && (long)N0 * nj * nk < (1L << 31)) {
+    // the LDS tile is 64 cells wide in x: narrow grids (N2 < 32) leave most
+    // of it idle and take the per-cell kernel; grid size below 2^32 threads
+    if (!msat_map && !aex_map && N2 >= kEK / 2 && (long)N0 * nj * nk * kBlock < (1L << 32)) {
         exchange6_lds_kernel<<<dim3((unsigned)(N0 * nj * nk)), kBlock, 0, as_stream(stream)>>>(
             hx, hy, hz, mx, my, mz, pre, N0, N1, N2, periodic0, periodic1, periodic2, c0, c1, c2,
             (int)nj, (int)nk);

@@ -168,8 +168,10 @@ template <typename C, int R, int DIR> struct OddDFT {
     {
         using T = decltype(v[0].x);
         C out[R];
+#pragma unroll
         for (int k = 0; k < R; k++) {
             T re = v[0].x, im = v[0].y;
+#pragma unroll
             for (int n = 1; n < R; n++) {
                 const int t = (n * k) % R;
                 const double ang = 2.0 * kPi * (double)t / (double)R;
@@ -180,6 +182,7 @@ template <typename C, int R, int DIR> struct OddDFT {
             }
             out[k] = C{re, im};
         }
+#pragma unroll
         for (int k = 0; k < R; k++) v[k] = out[k];
     }
 };

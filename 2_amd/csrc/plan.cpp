@@ -84,7 +84,8 @@ static int plan_init(DemagPlanImpl *P, int n0, int n1, int n2, const double cell
     P->cell_kernel_type = kernel_type;
     P->accuracy = accuracy;
     P->fast_x = fast_length(g.Lx) && fast_x_addressable(g);
-    P->fast_y = fast_length(g.p1);
+    // 3-D y passes take the longer odd lengths too; films run y2d
+    P->fast_y = (g.p0 == 1) ? fast_length(g.p1) : fast_ylength(g.p1);
     P->fast_z = g.p0 > 1 && fast_length(g.p0) && fast_z_addressable(g);
 
     // FFT descriptors

@@ -112,10 +112,72 @@ template <int N, int DIR, int Ns> struct R2Stages {
     }
 };
 
-// In-register FFT of N points (power of two), natural order in and out.
+// ------------------------------------- mixed radix (N = 2^a * {3, 5, 7})
+//
+// The same Stockham stage for radix R at span Ns: inputs a[J + r N/R],
+// r < R, times W_{Ns R}^{r k} (k = J mod Ns), an R-point DFT, outputs
+// b[(J/Ns) Ns R + k + r Ns].  The radix-2 stages run first (so that a line
+// whose upper half is zero starts with copies), the odd radix last.
+template <int N, int DIR, int Ns, int R> struct MRStage {
+    template <int J> __device__ __forceinline__ static void step(const float2 *a, float2 *b)
+    {
+        if constexpr (J < N / R) {
+            constexpr int k = J % Ns;
+            float2 v[R];
+            static_for_r<R>([&](auto r) {
+                constexpr int q = decltype(r)::value;
+                v[q] = twmul<Ns * R, k * q, DIR>(a[J + q * (N / R)]);
+            });
+            SmallDFT<float2, R, DIR>::run(v);
+            constexpr int base = (J / Ns) * Ns * R + k;
+            static_for_r<R>([&](auto r) {
+                constexpr int q = decltype(r)::value;
+                b[base + q * Ns] = v[q];
+            });
+            step<J + 1>(a, b);
+        }
+    }
+    template <typename F, int... I>
+    __device__ __forceinline__ static void static_for_r_impl(F &&f, std::integer_sequence<int, I...>)
+    {
+        (f(std::integral_constant<int, I>{}), ...);
+    }
+    template <int M, typename F> __device__ __forceinline__ static void static_for_r(F &&f)
+    {
+        static_for_r_impl(f, std::make_integer_sequence<int, M>{});
+    }
+};
+
+// the radix of the stage at span Ns: 2 while N/Ns is even, then the odd rest
+template <int N, int Ns> constexpr int mr_radix() { return ((N / Ns) % 2 == 0) ? 2 : N / Ns; }
+
+template <int N, int DIR, int Ns>
+__device__ __forceinline__ void mr_stages(float2 *a, float2 *b, bool &in_b)
+{
+    if constexpr (Ns < N) {
+        constexpr int R = mr_radix<N, Ns>();
+        static_assert(R == 2 || R == 3 || R == 5 || R == 7, "lengths 2^a * {1, 3, 5, 7} only");
+        MRStage<N, DIR, Ns, R>::template step<0>(a, b);
+        in_b = !in_b;
+        mr_stages<N, DIR, Ns * R>(b, a, in_b);
+    }
+}
+
+constexpr bool is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
+
+// In-register FFT of N points, natural order in and out: unrolled radix-2
+// Stockham for powers of two, mixed radix for N = 2^a * {3, 5, 7}.
 template <int N, int DIR> __device__ __forceinline__ void fft_reg(float2 (&v)[N])
 {
-    if constexpr (N == 1) {
+    if constexpr (!is_pow2(N)) {
+        float2 tmp[N];
+        bool in_b = false;
+        mr_stages<N, DIR, 1>(v, tmp, in_b);
+        if (in_b) {
+#pragma unroll
+            for (int i = 0; i < N; i++) v[i] = tmp[i];
+        }
+    } else if constexpr (N == 1) {
         return;
     } else if constexpr (N == 2) {
         const float2 a = v[0], b = v[1];
@@ -142,7 +204,22 @@ template <int N, int DIR> __device__ __forceinline__ void fft_reg(float2 (&v)[N]
 // registers are not read): the first radix-2 stage reduces to copies.
 template <int N, int DIR> __device__ __forceinline__ void fft_reg_halfin(float2 (&v)[N])
 {
-    if constexpr (N <= 8) {
+    if constexpr (!is_pow2(N)) {
+        // first radix-2 stage (Ns = 1) on a zero upper half: copies
+        static_assert(N % 2 == 0, "a zero-padded line has an even length");
+        float2 tmp[N];
+#pragma unroll
+        for (int j = 0; j < N / 2; j++) {
+            tmp[2 * j] = v[j];
+            tmp[2 * j + 1] = v[j];
+        }
+        bool in_b = true;
+        mr_stages<N, DIR, 2>(tmp, v, in_b);
+        if (in_b) {
+#pragma unroll
+            for (int i = 0; i < N; i++) v[i] = tmp[i];
+        }
+    } else if constexpr (N <= 8) {
 #pragma unroll
         for (int i = N / 2; i < N; i++) v[i] = float2{0.0f, 0.0f};
         fft_reg<N, DIR>(v);
@@ -189,12 +266,38 @@ template <> struct Split<256> { static constexpr int T = 16, E = 16; };
 template <> struct Split<512> { static constexpr int T = 16, E = 32; };
 template <> struct Split<1024> { static constexpr int T = 32, E = 32; };
 template <> struct Split<2048> { static constexpr int T = 32, E = 64; };
+// lengths with an odd factor 3, 5 or 7: the odd factor is in E (T must divide
+// E and be a power of two)
+template <> struct Split<24> { static constexpr int T = 2, E = 12; };
+template <> struct Split<48> { static constexpr int T = 4, E = 12; };
+template <> struct Split<96> { static constexpr int T = 4, E = 24; };
+template <> struct Split<192> { static constexpr int T = 8, E = 24; };
+template <> struct Split<384> { static constexpr int T = 8, E = 48; };
+template <> struct Split<40> { static constexpr int T = 2, E = 20; };
+template <> struct Split<80> { static constexpr int T = 4, E = 20; };
+template <> struct Split<160> { static constexpr int T = 4, E = 40; };
+template <> struct Split<320> { static constexpr int T = 8, E = 40; };
+template <> struct Split<56> { static constexpr int T = 2, E = 28; };
+template <> struct Split<112> { static constexpr int T = 4, E = 28; };
+template <> struct Split<224> { static constexpr int T = 4, E = 56; };
+template <> struct Split<448> { static constexpr int T = 8, E = 56; };
 
-inline bool fast_length_impl(int L)
+// Lengths of the register-FFT passes.  Strided y passes of 3-D grids take
+// every length above; the passes that hold three components per thread (x
+// passes, film y, and z whose LDS holds three components) only those with at
+// most 28 registers per component.
+inline bool fast_pow2_length(int L)
 {
     return L == 8 || L == 16 || L == 32 || L == 64 || L == 128 || L == 256 || L == 512 ||
            L == 1024 || L == 2048;
 }
+inline bool fast_odd_short(int L)
+{
+    return L == 24 || L == 48 || L == 96 || L == 192 || L == 40 || L == 80 || L == 56 || L == 112;
+}
+inline bool fast_odd_long(int L) { return L == 384 || L == 160 || L == 320 || L == 224 || L == 448; }
+inline bool fast_length_impl(int L) { return fast_pow2_length(L) || fast_odd_short(L); }
+inline bool fast_ylength_impl(int L) { return fast_length_impl(L) || fast_odd_long(L); }
 
 // LDS row pitch (in float2) of a column-tile transpose: rows of T*W points,
 // read back by lanes (t, w) at row t*G + g.  A pitch with G*S = W (mod 32)

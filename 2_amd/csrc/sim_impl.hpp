@@ -46,6 +46,7 @@ struct mmx_sim {
     bool dex = false;
     int dex_acc = 8;
     float *kern_dex = nullptr, *kern_fold = nullptr;
+    float *hdex = nullptr; // H_dex of the current stage when an Msat mask is set (3 N)
     bool dex_valid = false;
     double dex_ms = 0, dex_aex = 0;
     int dex_built_acc = 0;

@@ -1,15 +1,13 @@
-// xi_gen.hip -- the xi pass for power-of-two lengths with the general
+// xi_gen.hip -- the xi pass of the power-of-two lengths with the general
 // per-cell epilogue (Msat/Aex/alpha/Ku maps, the B_ext and anisU masks,
 // Zhang-Li with its polarisation and xi masks, the thermal field), in a file
 // of its own so that it compiles in parallel with demag_fast.hip.
-#include "fast_kernels.hpp"
+#include "fast_launch.hpp"
 
 namespace mmx {
 
-int fast_xi_general(const DemagPlanImpl *P, int mode, const FieldParams &fp,
-                    const EpilogueIO &io, hipStream_t st)
-{
-    return fast_xi_modes<true>(P, mode, fp, io, st);
-}
+#define MMX_INST_XI_GEN(L) MMX_INST_XI(L, true)
+MMX_POW2_LENGTHS(MMX_INST_XI_GEN)
+#undef MMX_INST_XI_GEN
 
 } // namespace mmx

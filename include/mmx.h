@@ -304,7 +304,11 @@ int mmx_sim_setmask(mmx_sim *sim, const char *name, const float *host,
 int mmx_sim_setarray(mmx_sim *sim, const char *name, const float *host,
                      int ncomp);
 int mmx_sim_getarray(mmx_sim *sim, const char *name, float *host, int ncomp);
-/* Device pointer of a quantity's storage (component c at ptr + c*N). */
+/* Device pointer of a quantity's storage (component c at ptr + c*N).  The
+ * pointer is valid until the next step, try or batch run (m and its work
+ * buffer swap on an accepted step) or the sim is destroyed; taking the "m"
+ * pointer marks m for normalisation at the next update (the caller may write
+ * through it). */
 int mmx_sim_device_array(mmx_sim *sim, const char *name, float **dev);
 /* The stream all of the sim's work is queued on (hipStream_t). */
 int mmx_sim_stream(mmx_sim *sim, void **stream);

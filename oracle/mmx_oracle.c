@@ -86,7 +86,7 @@ int orc_kernel_arne(const int p[3], const double cell[3], const int pbc[3],
     const int X = 0, Y = 1, Z = 2;
     if (!(p[0] > 0 && p[1] > 1 && p[2] > 1)) return -1;
     if (!(cell[0] > 0 && cell[1] > 0 && cell[2] > 0)) return -1;
-    if (accuracy_ <= 0) return -1;
+    if (accuracy_ < 0) return -1; /* 0: maxSize = d/0 = inf, one point per axis */
     if (p[1] % 2 || p[2] % 2) return -1;
     if (p[0] > 1 && p[0] % 2) return -1;
 

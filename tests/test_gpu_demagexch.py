@@ -121,10 +121,12 @@ def test_parameters_rebuild_the_kernel(mmx_gpu):
     assert relerr(s.getarray("H_dex"), D.h_dex(n, pbc, cell, 6e5, 2e-11, m, accuracy=3)) < TOL
     with pytest.raises(Exception):
         s.setv("demag_acc", 2.5)
-    # the kernel takes the Msat multiplier only: a mask is refused
+    # with an Msat mask H_dex takes its own convolution (the kernel keeps the
+    # multiplier); a mask of ones gives the folded convolution's field
+    h_folded = s.getarray("H_eff")
     s.setmask("Msat", np.ones(n, np.float32))
-    with pytest.raises(Exception, match="mask"):
-        s.getarray("H_eff")
+    h_masked = s.getarray("H_eff")
+    assert relerr(h_masked, h_folded.astype(np.float64)) < TOL
     s.close()
 
 
@@ -170,3 +172,34 @@ def test_reference_rk12_script(mmx_gpu):
     prob = O.Problem(n, cell, msat=800e3, aex=1.3e-11, alpha=0.02, bext=(0.0, 4.3e-3, -24.6e-3),
                      kernel=D.folded_khat(n, (0, 0, 0), cell, 800e3, 1.3e-11))
     assert relerr(s.getarray("H_eff"), prob.heff(m)) < TOL
+
+
+def test_heff_with_demagexch_and_msat_mask(mmx_gpu):
+    """With an Msat mask the reference keeps building the dex kernel from the
+    Msat multiplier and convolves m itself (mod_demagexchange.go:74-79,143),
+    while B takes Msat_map m: H_eff = H_demag(Msat_map m) + H_ex + H_dex(m)."""
+    n, cell, pbc = (2, 16, 32), (3e-9, 4e-9, 4e-9), (0, 0, 0)
+    _say("H_eff + H_dex with an Msat mask", n)
+    s, m = _sim(mmx_gpu, n, cell, pbc, seed=11)
+    ms = np.random.default_rng(12).uniform(0.5, 1.0, size=n).astype(np.float32)
+    s.setmask("Msat", ms, 1)
+    s.setv("B_ext", (0.0, 0.004, 0.009))
+    mo = O.normalize(m, ms)
+    prob = O.Problem(n, cell, pbc=pbc, msat=8e5, aex=1.3e-11, alpha=0.05,
+                     bext=(0.0, 0.004, 0.009), msat_map=ms)
+    want = prob.heff(mo) + D.h_dex(n, pbc, cell, 8e5, 1.3e-11, mo)
+    assert relerr(s.getarray("H_eff"), want) < TOL
+    assert relerr(s.getarray("torque"), prob.torque(mo, want)) < TOL
+    assert relerr(s.getarray("H_dex"), D.h_dex(n, pbc, cell, 8e5, 1.3e-11, mo)) < TOL
+    # one rk12 try: the predictor's H_dex comes from its own convolution
+    e = s.try_step(1e-14)
+    assert np.isfinite(e) and e > 0
+    s.close()
+
+
+def test_demag_acc_zero(mmx_gpu):
+    n, cell, pbc = (4, 16, 32), (2e-9, 3e-9, 5e-9), (0, 0, 0)
+    s, m = _sim(mmx_gpu, n, cell, pbc)
+    s.setv("demag_acc", 0)
+    assert relerr(s.getarray("H_dex"), D.h_dex(n, pbc, cell, 8e5, 1.3e-11, m, accuracy=0)) < TOL
+    s.close()

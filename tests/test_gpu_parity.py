@@ -192,7 +192,7 @@ def test_exchange6(torch_gpu, pbc, maps):
 
 @pytest.mark.parametrize("n,pbc", [((5, 9, 17), (0, 0, 0)), ((3, 8, 128), (1, 1, 1)),
                                    ((1, 6, 70), (0, 1, 0)), ((2, 1, 65), (1, 0, 0)),
-                                   ((4, 7, 1), (0, 0, 1))])
+                                   ((4, 7, 1), (0, 0, 1)), ((2, 5, 33), (0, 0, 1))])
 def test_exchange6_lds_path_is_bit_identical(torch_gpu, n, pbc):
     """Without maps mmx_exchange6 runs the LDS-staged stencil; with an Aex map
     of ones it runs the per-cell kernel whose weights are then exactly 1: the
@@ -409,7 +409,7 @@ def test_rk12_look_ahead_changes_nothing(torch_gpu):
         synth.setup_sim(sim, cfg)
         for _ in range(300):
             if defeat:
-                sim.device_array("m")
+                sim.device_array("H_eff")   # hands out a buffer: no look-ahead
             sim.step()
         runs.append((sim.getarray("m"), sim.getv("t"), sim.getv("dt"), sim.getv("badsteps"),
                      sim.getv("evals")))
@@ -506,3 +506,57 @@ def test_fixed_step_solvers(torch_gpu, solver):
     else:
         want = prob.normalized(mo)
     assert np.abs(got - want).max() < 1e-5
+
+
+# Transform lengths 2^a * {3, 5, 7} on the register path (mixed-radix
+# in-register FFTs): every x length (zero-padded and periodic), film y, 3-D y
+# up to 448 and z up to 56, against the oracle's convolution.
+RADIX_CASES = [((2, 12, 48), (0, 0, 0)), ((2, 40, 24), (0, 0, 0)), ((1, 28, 96), (0, 0, 0)),
+               ((1, 48, 40), (0, 0, 0)), ((12, 8, 16), (0, 0, 0)), ((20, 4, 8), (0, 0, 0)),
+               ((28, 4, 8), (0, 0, 0)), ((4, 96, 56), (0, 0, 0)), ((2, 192, 16), (0, 0, 0)),
+               ((2, 160, 16), (0, 0, 0)), ((2, 112, 16), (0, 0, 0)), ((2, 224, 16), (0, 0, 0)),
+               ((2, 80, 112), (0, 0, 0)), ((2, 20, 192), (0, 0, 0)), ((2, 56, 80), (0, 0, 0)),
+               ((4, 24, 96), (0, 1, 0)), ((4, 8, 48), (0, 0, 1)), ((24, 8, 8), (1, 0, 0)),
+               ((1, 40, 56), (0, 1, 0))]
+
+
+@pytest.mark.parametrize("n,pbc", RADIX_CASES)
+def test_demag_odd_lengths_register_path(torch_gpu, n, pbc):
+    import mmx
+    torch = torch_gpu
+    cell = (3e-9, 4e-9, 4e-9)
+    _say("odd-length demag B", n, pbc)
+    rng = np.random.default_rng(sum(n) + 3)
+    m = O.normalize(rng.normal(size=(3,) + n)).astype(np.float32)
+    ms_map = rng.uniform(0.5, 1.0, size=n).astype(np.float32)
+    plan = mmx.DemagPlan(n, cell, pbc)
+    p = plan.p
+    kh, _ = O.kernel_fft(p, K9=O.kernel_arne(p, cell, pbc))
+    for msat_map in (None, ms_map):
+        want = O.MU0 * 8e5 * O.demag_conv(n, p, kh, m.astype(np.float64), msat_map)
+        dm = torch.from_numpy(m).cuda()
+        db = torch.empty_like(dm)
+        dmap = None if msat_map is None else torch.from_numpy(msat_map).cuda()
+        plan.update_b(db.data_ptr(), dm.data_ptr(), 8e5, None if dmap is None else dmap.data_ptr())
+        torch.cuda.synchronize()
+        assert relerr(db.cpu().numpy(), want) < TOL
+
+
+@pytest.mark.parametrize("n", [(2, 24, 96), (1, 40, 80), (4, 56, 112)])
+def test_heff_odd_lengths_general_epilogue(torch_gpu, n):
+    """Zhang-Li with an Msat map at lengths with factors 3, 5, 7: the register
+    x pass with the general per-cell epilogue, H_eff, torque and RK tries."""
+    cell = (3e-9, 4e-9, 4e-9)
+    _say("odd-length H_eff/torque general epilogue", n)
+    sim, prob, m = _problem_pair(n, cell, anis=True, zl=True, bext=(0.0, 0.004, 0.009))
+    mo = prob.normalized(m.astype(np.float64).copy())
+    want_h = prob.heff(mo)
+    assert relerr(sim.getarray("H_eff"), want_h) < TOL
+    assert relerr(sim.getarray("torque"), prob.torque(mo, want_h)) < TOL
+    mo = m.astype(np.float64).copy()
+    for k in range(4):
+        dt = 1e-14 * (1 + 0.1 * k)
+        eg = sim.try_step(dt)
+        eo = prob.try_step(mo, dt)
+        assert abs(eg - eo) <= 1e-3 * eo + 1e-12
+    assert np.abs(sim.getarray("m") - prob.normalized(mo)).max() < TOL

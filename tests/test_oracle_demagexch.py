@@ -107,3 +107,19 @@ def test_dex_kernel_is_the_linear_combination():
     k6, _ = O.kernel_fft(p, K9=O.kernel_arne(p, CELL, pbc, 6))
     both = ms * O.demag_conv(n, p, k6, m) + hd
     assert np.abs(ms * O.demag_conv(n, p, fold, m) - both).max() <= 1e-9 * np.abs(both).max()
+
+
+def test_accuracy_zero_is_one_point_per_axis():
+    """demag_acc is a Uint (verify.go:42-47): 0 is allowed.  Kernel_Arne then
+    takes maxSize = d/0 = inf and one integration point per axis
+    (kernel_magnetostatic.go:107-119), which for cubic cells is also what
+    accuracy 1 gives (no displacement is shorter than the cell); for
+    non-cubic cells the self cell's point count differs."""
+    p = (4, 8, 8)
+    k0 = O.kernel_arne(p, (3e-9, 3e-9, 3e-9), (0, 0, 0), 0)
+    k1 = O.kernel_arne(p, (3e-9, 3e-9, 3e-9), (0, 0, 0), 1)
+    assert np.array_equal(k0, k1)
+    kn0 = O.kernel_arne(p, (2e-9, 3e-9, 5e-9), (0, 0, 0), 0)
+    kn1 = O.kernel_arne(p, (2e-9, 3e-9, 5e-9), (0, 0, 0), 1)
+    assert not np.array_equal(kn0, kn1)
+    assert np.isfinite(kn0).all()
