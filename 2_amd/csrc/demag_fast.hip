@@ -1,19 +1,13 @@
 // demag_fast.hip -- dispatch of the register-FFT demag passes on the plan's
-// transform lengths, and the launchers of the power-of-two lengths (the
-// kernels are in fast_kernels.hpp; the general xi epilogue of these lengths is
-// instantiated in xi_gen.hip, lengths with a factor 3, 5 or 7 in fast_r3.hip,
-// fast_r5.hip and fast_r7.hip).
-#include "fast_launch.hpp"
+// transform lengths.  The kernels are in fast_kernels.hpp, the launchers of
+// one length in fast_launch.hpp, instantiated by length in files that compile
+// in parallel: fast_px.hip (x passes), fast_py.hip (y), fast_pz.hip (z) and
+// xi_gen.hip (the general xi epilogue) for the powers of two, fast_r3.hip,
+// fast_r5.hip and fast_r7.hip for the lengths with a factor 3, 5 or 7.  This
+// file sees only the launchers' declarations, so it instantiates none.
+#include "fast_kernels.hpp"
 
 namespace mmx {
-
-MMX_POW2_LENGTHS(MMX_INST_XF)
-MMX_POW2_LENGTHS(MMX_INST_Y)
-MMX_POW2_LENGTHS(MMX_INST_Z)
-MMX_POW2_LENGTHS(MMX_INST_Y2D)
-#define MMX_INST_XI_PLAIN(L) MMX_INST_XI(L, false)
-MMX_POW2_LENGTHS(MMX_INST_XI_PLAIN)
-#undef MMX_INST_XI_PLAIN
 
 // ------------------------------------------------------------ host side
 

@@ -173,6 +173,157 @@ yi_fast(DemagGeom g, int ntx, const float2 *__restrict__ tw, const float2 *__res
     for (int q = 0; q < NQ; q++) st_off<float2>(dst + (size_t)q * C::T * g.sx, woff, v[q]);
 }
 
+// ------------------------------------------- y transforms of length 2048
+//
+// L = T E with T = 64 threads per column and E = L / 64 = 32 registers each
+// (the four-step split of ColTile needs T | E: 32 x 64, whose 64 registers and
+// 16 KB twiddle table leave room for three 32-thread columns per CU).  Step
+// one as in col_forward (thread t: E-point DFT of the rows t + T q, times
+// W_L^(t k1)); after the LDS exchange, the T-point DFT of each k1 runs on a
+// lane pair (p = k1, s): lane s takes the rows t = 2 m + s, an E-point DFT
+// Y_s, and the pair combines X[k1 + E k'] = Y_0 + W_T^k' Y_1 (lane 0) and
+// X[k1 + E (k' + T/2)] = Y_0 - W_T^k' Y_1 (lane 1) with one lane swap per
+// value.  Lanes: tid = s + 2 (w + W p), t = 2 p + s, so the pair is adjacent
+// (a DPP swap) and lanes of one row still cover W neighbouring columns.
+// Twiddles W_L^j = A[j / E] B[j % E] from two tables of T and E entries.
+
+template <int L> struct PairTile {
+    static constexpr int T = 64;
+    static constexpr int E = L / T;
+    static constexpr int W = 4;                // columns per workgroup
+    static constexpr int NT = T * W;           // 256 threads
+    // row pitch of the exchange: the 32 lanes of a b64 access hit distinct
+    // bank pairs in both directions (T W + 2 W = 2 W mod 32)
+    static constexpr int S = T * W + 2 * W;
+    static constexpr int LDS = (E * S + T + E) * 8;
+    static_assert(L == 2048 && E == T / 2, "pair split: E = T / 2");
+};
+
+// swap values between lanes 2i and 2i+1 (DPP quad_perm [1, 0, 3, 2])
+__device__ __forceinline__ float2 pair_swap(float2 v)
+{
+    float2 r;
+    r.x = __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v.x), 0xB1, 0xF, 0xF, false));
+    r.y = __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v.y), 0xB1, 0xF, 0xF, false));
+    return r;
+}
+
+// W_L^j (forward sign), 0 <= j < L
+template <int L>
+__device__ __forceinline__ float2 pair_tw(const float2 *sa, const float2 *sb, int j)
+{
+    constexpr int E = PairTile<L>::E;
+    return cmul(sa[j / E], sb[j % E]);
+}
+
+template <int L>
+__device__ __forceinline__ void pair_tables(const float2 *__restrict__ tw, float2 *sa, float2 *sb)
+{
+    using C = PairTile<L>;
+    for (int q = threadIdx.x; q < C::T; q += C::NT) sa[q] = tw[C::E * q];
+    for (int q = threadIdx.x; q < C::E; q += C::NT) sb[q] = tw[q];
+}
+
+// yf of length 2048 on the pair split; FULL and the zero-padded rows as yf_fast
+template <int L, bool FULL>
+__global__ void __launch_bounds__(PairTile<L>::NT)
+yf_pair(DemagGeom g, int ntx, const float2 *__restrict__ tw, const float2 *__restrict__ xb,
+        float2 *__restrict__ yb)
+{
+    using C = PairTile<L>;
+    constexpr int T = C::T, E = C::E, NQ = FULL ? E : E / 2;
+    extern __shared__ float2 sm[];
+    float2 *sbuf = sm, *sa = sm + E * C::S, *sb = sa + T;
+    const int tile = xcd_tile(blockIdx.x, gridDim.x);
+    const int tx = tile % ntx, rest = tile / ntx, i = rest % g.n0, c = rest / g.n0;
+    const int s = threadIdx.x & 1, w = (threadIdx.x >> 1) % C::W, p = (threadIdx.x >> 1) / C::W;
+    const int t = 2 * p + s;
+    pair_tables<L>(tw, sa, sb);
+    const int kx0 = tx * C::W;
+    const bool ok = kx0 + w < g.h2;
+    const int col = ok ? kx0 + w : g.h2 - 1; // past h2: load in bounds, store nothing
+    const float2 *src = xb + ((size_t)c * g.nrows + (size_t)i * g.n1) * g.sx;
+    const unsigned roff = (unsigned)(((size_t)t * g.sx + col) * sizeof(float2));
+    float2 v[E];
+#pragma unroll
+    for (int q = 0; q < NQ; q++) v[q] = ld_off<float2>(src + (size_t)q * T * g.sx, roff);
+    __syncthreads();
+    if constexpr (FULL)
+        fft_reg<E, -1>(v);
+    else
+        fft_reg_halfin<E, -1>(v);
+#pragma unroll
+    for (int k1 = 1; k1 < E; k1++) v[k1] = cmul(v[k1], pair_tw<L>(sa, sb, t * k1));
+#pragma unroll
+    for (int k1 = 0; k1 < E; k1++) sbuf[k1 * C::S + t * C::W + w] = v[k1];
+    __syncthreads();
+    float2 u[E];
+#pragma unroll
+    for (int m = 0; m < E; m++) u[m] = sbuf[p * C::S + (2 * m + s) * C::W + w];
+    fft_reg<E, -1>(u);
+#pragma unroll
+    for (int k = 0; k < E; k++) {
+        const float2 own = s ? cmul(u[k], sa[k]) : u[k]; // lane 1: W_T^k Y_1[k]
+        const float2 oth = pair_swap(own);
+        u[k] = s ? float2{oth.x - own.x, oth.y - own.y} : float2{own.x + oth.x, own.y + oth.y};
+    }
+    if (!ok) return;
+    float2 *dst = yb + ((size_t)c * g.n0 + i) * (size_t)g.p1 * g.sx;
+    const unsigned woff = (unsigned)(((size_t)(p + E * (T / 2) * s) * g.sx + col) * sizeof(float2));
+#pragma unroll
+    for (int k = 0; k < E; k++) st_off<float2>(dst + (size_t)E * k * g.sx, woff, u[k]);
+}
+
+// yi of length 2048 on the pair split: the steps of yf_pair reversed
+template <int L, bool FULL>
+__global__ void __launch_bounds__(PairTile<L>::NT)
+yi_pair(DemagGeom g, int ntx, const float2 *__restrict__ tw, const float2 *__restrict__ yb,
+        float2 *__restrict__ xb)
+{
+    using C = PairTile<L>;
+    constexpr int T = C::T, E = C::E, NQ = FULL ? E : E / 2;
+    extern __shared__ float2 sm[];
+    float2 *sbuf = sm, *sa = sm + E * C::S, *sb = sa + T;
+    const int tile = xcd_tile(blockIdx.x, gridDim.x);
+    const int tx = tile % ntx, rest = tile / ntx, i = rest % g.n0, c = rest / g.n0;
+    const int s = threadIdx.x & 1, w = (threadIdx.x >> 1) % C::W, p = (threadIdx.x >> 1) / C::W;
+    const int t = 2 * p + s;
+    pair_tables<L>(tw, sa, sb);
+    const int kx0 = tx * C::W;
+    const bool ok = kx0 + w < g.h2;
+    const int col = ok ? kx0 + w : g.h2 - 1;
+    const float2 *src = yb + ((size_t)c * g.n0 + i) * (size_t)g.p1 * g.sx;
+    const unsigned roff = (unsigned)(((size_t)(p + E * (T / 2) * s) * g.sx + col) * sizeof(float2));
+    float2 u[E]; // lane s: X[p + E (k + s T/2)]
+#pragma unroll
+    for (int k = 0; k < E; k++) u[k] = ld_off<float2>(src + (size_t)E * k * g.sx, roff);
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < E; k++) {
+        const float2 oth = pair_swap(u[k]);
+        // lane 0: X0 + X1; lane 1: (X0 - X1) W_T^-k
+        u[k] = s ? cmulc(float2{oth.x - u[k].x, oth.y - u[k].y}, sa[k])
+                 : float2{u[k].x + oth.x, u[k].y + oth.y};
+    }
+    fft_reg<E, +1>(u); // u[m]: row t = 2 m + s of step one's input, at k1 = p
+#pragma unroll
+    for (int m = 0; m < E; m++) {
+        const int tt = 2 * m + s;
+        const float2 x = (m == 0 && s == 0) ? u[m] : cmulc(u[m], pair_tw<L>(sa, sb, tt * p));
+        sbuf[p * C::S + tt * C::W + w] = x;
+    }
+    __syncthreads();
+    float2 v[E];
+#pragma unroll
+    for (int k1 = 0; k1 < E; k1++) v[k1] = sbuf[k1 * C::S + t * C::W + w];
+    fft_reg<E, +1>(v);
+    if (!ok) return;
+    float2 *dst = xb + ((size_t)c * g.nrows + (size_t)i * g.n1) * g.sx;
+    const unsigned woff = (unsigned)(((size_t)t * g.sx + col) * sizeof(float2));
+#pragma unroll
+    for (int q = 0; q < NQ; q++) st_off<float2>(dst + (size_t)q * T * g.sx, woff, v[q]);
+}
+
 // z pass: three thread groups, one per field component, each transforms its
 // component of W spectrum columns along z; the 3x3 kernel multiply runs
 // through LDS, one octant kernel point for each pair kz, p0-kz.

@@ -34,6 +34,16 @@ int xf_len(const DemagPlanImpl *P, int pred, const float *m, const float *msat_m
 template <int L> int yf_len(const DemagPlanImpl *P, hipStream_t st)
 {
     const DemagGeom &g = P->g;
+    if constexpr (L == 2048) {
+        using C = PairTile<L>;
+        const int ntx = div_up(g.h2, C::W);
+        auto kern = (g.n1 == g.p1) ? yf_pair<L, true> : yf_pair<L, false>;
+        allow_lds_fast(kern, C::LDS);
+        kern<<<dim3(ntx * g.n0 * 3), dim3(C::NT), C::LDS, st>>>(g, ntx, P->fy.tw, P->xbuf,
+                                                                 P->ybuf);
+        MMX_CHECK_LAUNCH();
+        return MMX_OK;
+    }
     using C = ColTile<L>;
     const int ntx = div_up(g.h2, C::W);
     auto kern = (g.n1 == g.p1) ? yf_fast<L, true> : yf_fast<L, false>;
@@ -46,6 +56,16 @@ template <int L> int yf_len(const DemagPlanImpl *P, hipStream_t st)
 template <int L> int yi_len(const DemagPlanImpl *P, hipStream_t st)
 {
     const DemagGeom &g = P->g;
+    if constexpr (L == 2048) {
+        using C = PairTile<L>;
+        const int ntx = div_up(g.h2, C::W);
+        auto kern = (g.n1 == g.p1) ? yi_pair<L, true> : yi_pair<L, false>;
+        allow_lds_fast(kern, C::LDS);
+        kern<<<dim3(ntx * g.n0 * 3), dim3(C::NT), C::LDS, st>>>(g, ntx, P->fy.tw, P->ybuf,
+                                                                 P->xbuf);
+        MMX_CHECK_LAUNCH();
+        return MMX_OK;
+    }
     using C = ColTile<L>;
     const int ntx = div_up(g.h2, C::W);
     auto kern = (g.n1 == g.p1) ? yi_fast<L, true> : yi_fast<L, false>;

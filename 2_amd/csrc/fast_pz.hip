@@ -1,0 +1,10 @@
+// fast_pz.hip -- the z-pass launchers of the
+// power-of-two lengths (fast_launch.hpp), in a file of its own so that it
+// compiles in parallel with the other instantiations.
+#include "fast_launch.hpp"
+
+namespace mmx {
+
+MMX_POW2_LENGTHS(MMX_INST_Z)
+
+} // namespace mmx

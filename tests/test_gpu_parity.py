@@ -99,6 +99,10 @@ CONV_CASES = [((1, 32, 128), (3e-9, 3.90625e-9, 3.90625e-9), (0, 0, 0)),
               ((1, 1024, 8), (3e-9, 4e-9, 4e-9), (0, 0, 0)),
               # the longest transforms: y 2048 in 3-D, x half-lengths 1024 and 2048
               ((2, 1024, 24), (2e-9, 2e-9, 2e-9), (0, 0, 0)),
+              # y 2048 on the lane-pair split: under a register z pass, and
+              # periodic (every row holds data)
+              ((8, 1024, 4), (2e-9, 2e-9, 3e-9), (0, 0, 0)),
+              ((2, 2048, 4), (3e-9, 2e-9, 2e-9), (0, 1, 0)),
               ((2, 12, 1024), (2e-9, 2e-9, 2e-9), (0, 0, 0)),
               ((1, 8, 2048), (3e-9, 4e-9, 4e-9), (0, 0, 0)),
               # z transforms of length 128, 512, 2048 (zero-padded half), and
