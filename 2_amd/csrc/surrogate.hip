@@ -12,8 +12,10 @@
 //               6*n0, 6*n0+1 = the position in the coarse cell (+-1 for odd /
 //               even y, x: the residual operator has period 2), in bf16;
 //   net       = 4 conv 3x3 layers 32 -> 32 -> 32 -> 32 -> 16 (ReLU after the
-//               first three), BF16 MFMA (v_mfma_f32_16x16x16_bf16) with FP32
-//               accumulation, as implicit GEMMs over 16 x 8 pixel tiles;
+//               first three), BF16 MFMA with FP32 accumulation (gfx950's
+//               v_mfma_f32_16x16x32_bf16, 32 channels or pixels per
+//               instruction; 16-channel inputs on v_mfma_f32_16x16x16_bf16),
+//               as implicit GEMMs over 16 x 8 pixel tiles;
 //   B_fine   ~= upsampled coarse B + mu0 Ms * out[c*n0+i].
 //
 // Training (on the GPU, by these kernels): relative squared error of the
@@ -40,7 +42,19 @@ namespace mmx {
 
 typedef unsigned short bf16_t;
 typedef short sh4 __attribute__((ext_vector_type(4)));
+typedef short sh8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 typedef float fl4 __attribute__((ext_vector_type(4)));
+
+// D += A B over k = 32 on gfx950's double-K bf16 MFMA: lane (n = lane % 16,
+// g = lane / 16) passes 8 values of row n of A and 8 of column n of B, and
+// holds D[4 g + e][n], e < 4.  Both operands index their k by the same
+// (g, element), so the sum covers each k once whatever order the unit uses.
+__device__ __forceinline__ fl4 mfma_k32(sh8 a, sh8 b, fl4 c)
+{
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf8, a),
+                                                   __builtin_bit_cast(bf8, b), c, 0, 0, 0);
+}
 
 constexpr int kSurCh[5] = {32, 32, 32, 32, 16};
 
@@ -193,7 +207,7 @@ sur_conv(const bf16_t *__restrict__ in, const bf16_t *__restrict__ wt,
     }
     __syncthreads();
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int n = lane & 15, kq = (lane >> 4) * 4;
+    const int n = lane & 15, kq = (lane >> 4) * 4, kq8 = (lane >> 4) * 8;
 #pragma unroll
     for (int r = 0; r < 2; r++) {
         const int row = wave * 2 + r;
@@ -203,13 +217,28 @@ sur_conv(const bf16_t *__restrict__ in, const bf16_t *__restrict__ wt,
 #pragma unroll
         for (int tap = 0; tap < 9; tap++) {
             const int dy = tap / 3, dx = tap % 3;
+            const bf16_t *bp = tile + ((row + dy) * 18 + n + dx) * CP;
+            if constexpr (CIN % 32 == 0) { // 32 input channels per MFMA
 #pragma unroll
-            for (int kb = 0; kb < CIN; kb += 16) {
-                const sh4 b = *reinterpret_cast<const sh4 *>(tile + ((row + dy) * 18 + n + dx) * CP + kb + kq);
+                for (int kb = 0; kb < CIN; kb += 32) {
+                    const sh8 b = *reinterpret_cast<const sh8 *>(bp + kb + kq8);
 #pragma unroll
-                for (int mt = 0; mt < COUT / 16; mt++) {
-                    const sh4 a = *reinterpret_cast<const sh4 *>(wsh + (tap * COUT + mt * 16 + n) * CP + kb + kq);
-                    acc[mt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[mt], 0, 0, 0);
+                    for (int mt = 0; mt < COUT / 16; mt++) {
+                        const sh8 a = *reinterpret_cast<const sh8 *>(
+                            wsh + (tap * COUT + mt * 16 + n) * CP + kb + kq8);
+                        acc[mt] = mfma_k32(a, b, acc[mt]);
+                    }
+                }
+            } else { // 16 channels (the last layer's backward-data input)
+#pragma unroll
+                for (int kb = 0; kb < CIN; kb += 16) {
+                    const sh4 b = *reinterpret_cast<const sh4 *>(bp + kb + kq);
+#pragma unroll
+                    for (int mt = 0; mt < COUT / 16; mt++) {
+                        const sh4 a = *reinterpret_cast<const sh4 *>(
+                            wsh + (tap * COUT + mt * 16 + n) * CP + kb + kq);
+                        acc[mt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[mt], 0, 0, 0);
+                    }
                 }
             }
         }
@@ -274,45 +303,89 @@ sur_conv_wgrad(const bf16_t *__restrict__ x, const bf16_t *__restrict__ dz, floa
     }
     __syncthreads();
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int n = lane & 15, kq = (lane >> 4) * 4;
+    const int n = lane & 15, kq = (lane >> 4) * 4, kq8 = (lane >> 4) * 8;
     constexpr int MT = COUT / 16, NT = CIN / 16, ITEMS = 9 * MT * NT;
     float *dst = part + tileid * (long)(9 * CIN * COUT + COUT);
     for (int it = wave; it < ITEMS; it += 4) {
         const int tap = it / (MT * NT), mt = (it / NT) % MT, nt = it % NT;
         const int dy = tap / 3, dx = tap % 3;
         fl4 acc = fl4{0.0f, 0.0f, 0.0f, 0.0f};
+        // k = the tile's 256 pixels, 32 per MFMA
 #pragma unroll 4
-        for (int ks = 0; ks < 16; ks++) {
-            sh4 a, b;
+        for (int ks = 0; ks < 8; ks++) {
+            sh8 a, b;
 #pragma unroll
-            for (int e = 0; e < 4; e++) {
-                const int p = ks * 16 + kq + e, py = p >> 4, px = p & 15;
+            for (int e = 0; e < 8; e++) {
+                const int p = ks * 32 + kq8 + e, py = p >> 4, px = p & 15;
                 a[e] = (short)dzt[p * DP + mt * 16 + n];
                 b[e] = (short)xt[((py + dy) * 18 + px + dx) * XP + nt * 16 + n];
             }
-            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+            acc = mfma_k32(a, b, acc);
         }
         // D[m = co][n = ci]: lane holds co = mt*16 + kq + e, ci = nt*16 + n
 #pragma unroll
         for (int e = 0; e < 4; e++)
             dst[(tap * CIN + nt * 16 + n) * COUT + mt * 16 + kq + e] = acc[e];
     }
-    if (threadIdx.x < COUT) {
+    // bias gradient: wave w sums pixels 64 w .. 64 w + 63 of channel `lane`,
+    // then the four partial sums in wave order
+    __shared__ float bsum[4][COUT];
+    if (lane < COUT) {
         float sum = 0.0f;
-        for (int p = 0; p < 256; p++) sum += bf2f(dzt[p * DP + threadIdx.x]);
-        dst[9 * CIN * COUT + threadIdx.x] = sum;
+#pragma unroll 8
+        for (int p = 64 * wave; p < 64 * wave + 64; p++) sum += bf2f(dzt[p * DP + lane]);
+        bsum[wave][lane] = sum;
+    }
+    __syncthreads();
+    if (threadIdx.x < COUT)
+        dst[9 * CIN * COUT + threadIdx.x] =
+            ((bsum[0][threadIdx.x] + bsum[1][threadIdx.x]) + bsum[2][threadIdx.x]) +
+            bsum[3][threadIdx.x];
+}
+
+// grad[e] = sum over tiles of part[tile][e] in a fixed order, with the
+// tiles split over the 16 waves of a workgroup: wave g sums tiles
+// [g per, (g + 1) per) for 64 consecutive elements, then wave 0 adds the 16
+// partial sums in wave order (deterministic; one serial chain per element
+// over all tiles was latency-bound: 98 us per layer at 256 tiles).
+constexpr int kRedGroups = 16;
+__global__ void __launch_bounds__(64 * kRedGroups)
+sur_reduce_tiles(const float *__restrict__ part, int ntile, int len, float *__restrict__ grad)
+{
+    __shared__ float ps[kRedGroups][64];
+    const int lane = threadIdx.x & 63, grp = threadIdx.x >> 6;
+    const int e = blockIdx.x * 64 + lane;
+    const int per = (ntile + kRedGroups - 1) / kRedGroups;
+    const int t0 = grp * per, t1 = min(ntile, t0 + per);
+    float s = 0.0f;
+    if (e < len) {
+#pragma unroll 4
+        for (int t = t0; t < t1; t++) s += part[(long)t * len + e];
+    }
+    ps[grp][lane] = s;
+    __syncthreads();
+    if (grp == 0 && e < len) {
+        float r = 0.0f;
+        for (int g = 0; g < kRedGroups; g++) r += ps[g][lane];
+        grad[e] = r;
     }
 }
 
-// grad[e] = sum over tiles of part[tile][e], fixed order
-__global__ void __launch_bounds__(kBlock)
-sur_reduce_parts(const float *__restrict__ part, int ntile, int len, float *__restrict__ grad)
+// out[0] = sum of the n values of part in a fixed order: thread i of one
+// workgroup sums i, i + 1024, ..., then a tree in LDS (deterministic).
+__global__ void __launch_bounds__(1024)
+sur_reduce_one(const float *__restrict__ part, int n, float *__restrict__ out)
 {
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= len) return;
+    __shared__ float ps[1024];
     float s = 0.0f;
-    for (int t = 0; t < ntile; t++) s += part[(long)t * len + e];
-    grad[e] = s;
+    for (int t = threadIdx.x; t < n; t += 1024) s += part[t];
+    ps[threadIdx.x] = s;
+    __syncthreads();
+    for (int h = 512; h > 0; h >>= 1) {
+        if (threadIdx.x < h) ps[threadIdx.x] += ps[threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = ps[0];
 }
 
 __device__ __forceinline__ float target_at(const SurGeom &s, const float *__restrict__ bfine,
@@ -554,7 +627,7 @@ int net_backward(mmx_surrogate *S, const float *tgt_direct, double *loss, hipStr
     const int nlb = (int)div_up(n16, kBlock);
     float *t2 = S->loss_part + nlb; // sum target^2 (one float after the partials)
     sur_target_sq<<<dim3(nlb), dim3(kBlock), 0, st>>>(S->s, S->bfine, S->bc, tgt_direct, S->loss_part);
-    sur_reduce_parts<<<dim3(1), dim3(kBlock), 0, st>>>(S->loss_part, nlb, 1, t2);
+    sur_reduce_one<<<dim3(1), dim3(1024), 0, st>>>(S->loss_part, nlb, t2);
     sur_loss<<<dim3(nlb), dim3(kBlock), 0, st>>>(S->s, S->out, S->bfine, S->bc, tgt_direct, t2,
                                                  S->gout, S->loss_part);
     const dim3 grid(S->W / 16, S->H / 8), blk(kBlock);
@@ -564,14 +637,15 @@ int net_backward(mmx_surrogate *S, const float *tgt_direct, double *loss, hipStr
     // layer 3 (32 -> 16): weight gradient, then d act[2]
     sur_conv_wgrad<32, 16><<<wgrid, blk, 0, st>>>(xin[3], S->gout, S->part, S->H, S->W);
     int len = 9 * 32 * 16 + 16;
-    sur_reduce_parts<<<dim3(div_up(len, kBlock)), blk, 0, st>>>(S->part, ntile, len, S->grads + S->off_w[3]);
+    sur_reduce_tiles<<<dim3(div_up(len, 64)), dim3(64 * kRedGroups), 0, st>>>(
+        S->part, ntile, len, S->grads + S->off_w[3]);
     sur_conv<16, 32, CE_MASK_BF16, 0><<<grid, blk, 0, st>>>(S->gout, S->wbt[3], nullptr, S->act[2],
                                                             S->dz[2], S->H, S->W, S->s, nullptr, nullptr);
     for (int l = 2; l >= 0; l--) {
         sur_conv_wgrad<32, 32><<<wgrid, blk, 0, st>>>(xin[l], S->dz[l], S->part, S->H, S->W);
         len = 9 * 32 * 32 + 32;
-        sur_reduce_parts<<<dim3(div_up(len, kBlock)), blk, 0, st>>>(S->part, ntile, len,
-                                                                    S->grads + S->off_w[l]);
+        sur_reduce_tiles<<<dim3(div_up(len, 64)), dim3(64 * kRedGroups), 0, st>>>(
+            S->part, ntile, len, S->grads + S->off_w[l]);
         if (l > 0)
             sur_conv<32, 32, CE_MASK_BF16, 0><<<grid, blk, 0, st>>>(
                 S->dz[l], S->wbt[l], nullptr, S->act[l - 1], S->dz[l - 1], S->H, S->W, S->s, nullptr,

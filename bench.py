@@ -407,6 +407,44 @@ def surrogate_costs(reps=200):
                                      "GPU, then 1 ns exact and surrogate-assisted runs)"}
 
 
+def run_surrogate_kernels(args):
+    """The surrogate's kernels alone at the learned-correction configuration's
+    size (film 256x256x4), for profiling them: `--steps` training steps (the
+    exact target field, net forward, backward, weight gradients, Adam) and as
+    many corrector-field predictions (coarse field + net) on a seeded texture,
+    each timed over a device synchronisation."""
+    import torch
+    import mmx
+    from mmx import synth
+
+    cfg = synth.config(5)
+    n, cell = cfg["n"], cfg["cell"]
+    x = mmx.Sim(n, cell)
+    synth.setup_sim(x, cfg)
+    sur = mmx.Surrogate(x, seed=5)
+    x.setarray("m", synth.training_texture(1, n, cell))
+    m_ptr = x.device_array("m")
+    dm = torch.from_numpy(x.getarray("m")).cuda()
+    bb = torch.empty_like(dm)
+    sur.train_step(m_ptr, 1e-4)
+    sur.predict_b(dm.data_ptr(), bb.data_ptr())
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        sur.train_step(m_ptr, 1e-4)
+    torch.cuda.synchronize()
+    t_train = (time.perf_counter() - t0) / args.steps
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        sur.predict_b(dm.data_ptr(), bb.data_ptr())
+    torch.cuda.synchronize()
+    t_pred = (time.perf_counter() - t0) / args.steps
+    return {"workload": "surrogate kernels, film 256x256x4 cells of 4x4x3 nm", "steps": args.steps,
+            "train_step_us": 1e6 * t_train, "predict_us": 1e6 * t_pred,
+            "train_step_includes": "exact target field, forward, backward-data, weight "
+                                   "gradients, Adam"}
+
+
 def run_surrogate(args):
     """The learned-correction configuration: 256x256x4 film of 4x4x3 nm cells,
     Ms 8e5, A 1.3e-11, alpha 0.02, seeded random m.  The net is trained on the
@@ -653,7 +691,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--rk-steps", type=int, default=1000)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=("bulk", "precision", "batch", "surrogate", "demagexch"),
+    ap.add_argument("--workload", choices=("bulk", "precision", "batch", "surrogate",
+                                           "surrogate-kernels", "demagexch"),
                     default="bulk",
                     help="bulk: 256x256x64 (the headline line); precision: 1024x1024x32 film "
                          "with the FP64 Newell kernel and per-field errors against the FP64 "
@@ -688,6 +727,9 @@ def main():
         return
     if args.workload == "surrogate":
         print(json.dumps(run_surrogate(args)))
+        return
+    if args.workload == "surrogate-kernels":
+        print(json.dumps(run_surrogate_kernels(args)))
         return
     if args.workload == "demagexch":
         print(json.dumps(run_demagexch(args)))
