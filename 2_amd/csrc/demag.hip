@@ -198,10 +198,12 @@ int demag_forward(DemagPlanImpl *P, const float *m, const float *msat_map, hipSt
     if (P->fast_x) {
         MMX_TRY(fast_xf(P, 0, m, msat_map, nullptr, 0.0f, nullptr, nullptr, st));
     } else {
+        MMX_REQUIRE(!P->handoff_dev, MMX_ERR_ARG, "demag_forward: hand-off needs the fast x pass");
         xf_kernel<0><<<dim3(nb), dim3(kBlock), b, st>>>(g, P->fx, P->tiles.rows_x, m, msat_map,
                                                          nullptr, 0.0f, nullptr, nullptr, P->xbuf);
         MMX_CHECK_LAUNCH();
     }
+    if (P->after_x) MMX_HIP(hipEventRecord(P->after_x, st));
     mark(P, st, PASS_XF);
     return y_and_z_passes(P, st);
 }

@@ -96,6 +96,12 @@ struct DemagPlanImpl {
     hipEvent_t *events = nullptr; // when set: recorded before/after every launch
     int *event_pass = nullptr;    // pass id of the launch that each event ends
     int nevent = 0;
+    // Set by the caller around one demag_forward on the register-FFT path:
+    // the x pass's first thread moves the word *handoff_dev (left by the
+    // previous launch on the stream) into the pinned host word *handoff_host
+    // and clears it, and after_x is recorded once the x pass has been queued.
+    unsigned int *handoff_dev = nullptr, *handoff_host = nullptr;
+    hipEvent_t after_x = nullptr;
     // ConvPlan: the real-space kernel comes from the caller (6 host float32
     // arrays of the padded size, NULL = zero component), every component is
     // transformed and the realness ratio is recorded, not enforced

@@ -612,9 +612,16 @@ __global__ void __launch_bounds__(RowTile<LX>::NT)
 xf_fast(DemagGeom g, const float2 *__restrict__ tw, const float2 *__restrict__ rtw,
         const float *__restrict__ m, const float *__restrict__ msat_map,
         const float *__restrict__ tau0, float h, const float *__restrict__ norm_map,
-        float *__restrict__ mpred, float2 *__restrict__ xb)
+        float *__restrict__ mpred, float2 *__restrict__ xb, unsigned int *ho_dev,
+        unsigned int *ho_host)
 {
     using C = RowTile<LX>;
+    if (ho_dev != nullptr && blockIdx.x == 0 && threadIdx.x == 0) {
+        // the previous launch's result word to the host, cleared for the next
+        const unsigned int e = __hip_atomic_load(ho_dev, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(ho_host, e, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        __hip_atomic_store(ho_dev, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
     constexpr int NQ = FULL ? C::E : C::E / 2; // cell pairs 2 (t + T q), q < NQ
     extern __shared__ float2 sm[];
     float2 *sbuf = sm, *stw = sm + C::E * C::S, *srtw = stw + LX;

@@ -20,12 +20,13 @@ int xf_len(const DemagPlanImpl *P, int pred, const float *m, const float *msat_m
         auto kern = full ? xf_fast<L, 1, true> : xf_fast<L, 1, false>;
         allow_lds_fast(kern, C::LDS);
         kern<<<dim3(nb), dim3(C::NT), C::LDS, st>>>(g, P->fx.tw, P->fx.rtw, m, msat_map, tau0, h,
-                                                     norm_map, mpred, P->xbuf);
+                                                     norm_map, mpred, P->xbuf, nullptr, nullptr);
     } else {
         auto kern = full ? xf_fast<L, 0, true> : xf_fast<L, 0, false>;
         allow_lds_fast(kern, C::LDS);
         kern<<<dim3(nb), dim3(C::NT), C::LDS, st>>>(g, P->fx.tw, P->fx.rtw, m, msat_map, nullptr,
-                                                     0.0f, nullptr, nullptr, P->xbuf);
+                                                     0.0f, nullptr, nullptr, P->xbuf,
+                                                     P->handoff_dev, P->handoff_host);
     }
     MMX_CHECK_LAUNCH();
     return MMX_OK;

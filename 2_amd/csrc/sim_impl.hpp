@@ -69,6 +69,7 @@ struct mmx_sim {
     bool spec_valid = false;
     hipEvent_t ev_err = nullptr;
     unsigned int *derr = nullptr; // [0]: RK error; [4 .. 12]: implicit-solver maxima
+    bool derr_clear = false;      // derr[0] is known to be zero (cleared by a hand-off)
     unsigned int *herr = nullptr; // pinned, same layout
     hipStream_t st = nullptr;
 
