@@ -484,7 +484,7 @@ static int make_line_fft_f64(int L, LineFFT<double> &f, std::vector<double2> &ho
 {
     f.L = L;
     if (!factor_radices(L, f.radix, &f.nstages)) {
-        set_error("FFT length %d has a prime factor > 7", L);
+        set_error("FFT length %d has more than %d prime factors", L, kMaxStages);
         return MMX_ERR_SIZE;
     }
     host.resize(L);

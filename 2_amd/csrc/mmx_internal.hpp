@@ -229,6 +229,48 @@ __device__ inline void stockham_stage(const C *__restrict__ A, C *__restrict__ B
     }
 }
 
+// The same stage for a prime radix R > 7 known only at run time: each output
+// of a butterfly is its R-point DFT evaluated directly from the twiddled
+// inputs in A (R^2 complex multiply-adds per butterfly, the DFT's roots
+// W_R^(rq) = W_L^((rq mod R) L/R) taken from the same table), so that every
+// transform length has a plan, as cuFFT gives the reference
+// (fftplanX.go:59-64).
+template <typename C, int DIR>
+__device__ inline void stockham_stage_any(const C *__restrict__ A, C *__restrict__ B, int nl,
+                                          int ls, int es, int L, int Ns, int R,
+                                          const C *__restrict__ tw)
+{
+    const int M = L / R;
+    const int nb = nl * M;
+    const int tstep = L / (Ns * R);
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+        const int l = b / M;
+        const int j = b - l * M;
+        const int k = j % Ns;
+        const C *a = A + l * ls;
+        const int base = (j / Ns) * Ns * R + k;
+        C *o = B + l * ls;
+        for (int q = 0; q < R; q++) {
+            C acc = a[j * es];
+            int rq = 0; // r q mod R
+            for (int r = 1; r < R; r++) {
+                rq += q;
+                if (rq >= R) rq -= R;
+                C v = a[(j + r * M) * es];
+                if (Ns > 1) {
+                    const C w = tw[r * k * tstep];
+                    v = (DIR < 0) ? cmul(v, w) : cmulc(v, w);
+                }
+                const C w = tw[rq * M];
+                v = (DIR < 0) ? cmul(v, w) : cmulc(v, w);
+                acc.x += v.x;
+                acc.y += v.y;
+            }
+            o[(base + q * Ns) * es] = acc;
+        }
+    }
+}
+
 // Full transform of nl lines; ping-pongs between A and B and returns the
 // buffer that holds the result.  Ends with a barrier.
 template <typename T, int DIR>
@@ -247,7 +289,8 @@ __device__ inline typename cpx_t<T>::type *lds_fft(typename cpx_t<T>::type *A,
         case 2: stockham_stage<C, 2, DIR>(A, B, nl, ls, es, f.L, Ns, f.tw); break;
         case 3: stockham_stage<C, 3, DIR>(A, B, nl, ls, es, f.L, Ns, f.tw); break;
         case 5: stockham_stage<C, 5, DIR>(A, B, nl, ls, es, f.L, Ns, f.tw); break;
-        default: stockham_stage<C, 7, DIR>(A, B, nl, ls, es, f.L, Ns, f.tw); break;
+        case 7: stockham_stage<C, 7, DIR>(A, B, nl, ls, es, f.L, Ns, f.tw); break;
+        default: stockham_stage_any<C, DIR>(A, B, nl, ls, es, f.L, Ns, R, f.tw); break;
         }
         __syncthreads();
         C *t = A;

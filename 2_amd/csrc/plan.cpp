@@ -29,6 +29,11 @@ bool factor_radices(int L, int *radix, int *nstages)
     const int odd[3] = {3, 5, 7};
     for (int q = 0; q < 3; q++)
         while (rem % odd[q] == 0 && n < kMaxStages) { radix[n++] = odd[q]; rem /= odd[q]; }
+    // any other prime: a stage of the direct-DFT kind (stockham_stage_any)
+    for (int p = 11; rem > 1 && n < kMaxStages; p += 2) {
+        if ((long)p * p > rem) p = rem; // rem itself is prime
+        while (rem % p == 0 && n < kMaxStages) { radix[n++] = p; rem /= p; }
+    }
     *nstages = n;
     return rem == 1;
 }
@@ -96,8 +101,8 @@ static int plan_init(DemagPlanImpl *P, int n0, int n1, int n2, const double cell
     for (int a = 0; a < 3; a++) {
         fs[a]->L = lens[a];
         MMX_REQUIRE(factor_radices(lens[a], fs[a]->radix, &fs[a]->nstages), MMX_ERR_SIZE,
-                    "demag plan: FFT length %d has a prime factor > 7 (use sizes 2^n * {1,3,5,7})",
-                    lens[a]);
+                    "demag plan: FFT length %d has more than %d prime factors", lens[a],
+                    kMaxStages);
         twiddles_f32(lens[a], tw[a], rtw[a]);
     }
     // layout: tw x | rtw x | tw y | tw z

@@ -117,7 +117,13 @@ CONV_CASES = [((1, 32, 128), (3e-9, 3.90625e-9, 3.90625e-9), (0, 0, 0)),
               ((4, 32, 16), (3e-9, 4e-9, 4e-9), (0, 1, 0)),
               # periodic y of length 6 under a register-FFT z pass (tile pairs
               # ky, p1 - ky around p1/2 = 3)
-              ((4, 6, 8), (3e-9, 4e-9, 4e-9), (0, 1, 0))]
+              ((4, 6, 8), (3e-9, 4e-9, 4e-9), (0, 1, 0)),
+              # prime factors above 7 (the direct-DFT Stockham stage): x half
+              # lengths 13 and 31, y 22 and 34, z 6 with y 22, periodic y 34
+              ((1, 11, 13), (3e-9, 4e-9, 4e-9), (0, 0, 0)),
+              ((3, 11, 26), (3e-9, 4e-9, 4e-9), (0, 0, 0)),
+              ((2, 17, 62), (3e-9, 3e-9, 3e-9), (0, 0, 0)),
+              ((2, 34, 12), (3e-9, 3e-9, 3e-9), (0, 1, 0))]
 
 
 @pytest.mark.parametrize("n,cell,pbc", CONV_CASES)
@@ -434,7 +440,7 @@ def test_step_counts_and_errors(torch_gpu):
     with pytest.raises(mmx.MMXError):
         sim.load("nonsense")
     with pytest.raises(mmx.MMXError):
-        mmx.DemagPlan((1, 8, 11), (1e-9,) * 3)   # padded 22 -> FFT 11: prime > 7
+        mmx.DemagPlan((1, 8, 0), (1e-9,) * 3)    # empty axis
 
 
 
