@@ -407,6 +407,76 @@ def surrogate_costs(reps=200):
                                      "GPU, then 1 ns exact and surrogate-assisted runs)"}
 
 
+def run_film(args):
+    """The plain method on the small film (the reference's own CPU-runnable
+    case): 128x32x1 cells of 3.90625x3.90625x3 nm, Ms 8e5, A 1.3e-11, alpha
+    0.02, seeded random m, 10 mT along a seeded in-plane direction, demag +
+    exchange, RK12 at m_maxerror 1e-5, FP32, run 1 ns with <m> every 10 ps;
+    reports simulated ns/s, steps and the final H_eff against the FP64 oracle
+    at the same m.  cpu_baseline: the oracle's RK12 (FP64, same controller)
+    from the same state for about 10 s."""
+    import numpy as np
+    import torch
+    import mmx
+    from mmx import synth
+    from oracle import oracle as O
+
+    cfg = synth.config(1)
+    n, cell = cfg["n"], cfg["cell"]
+    s = mmx.Sim(n, cell)
+    synth.setup_sim(s, cfg)
+    s.setarray("m", cfg["m"])
+    s.update("H_eff")                  # kernel construction
+    torch.cuda.synchronize()
+    t0, step0 = s.getv("t"), s.getv("step")
+    table = []
+    ts = time.perf_counter()
+    for q in range(1, 101):            # <m> every 10 ps
+        s.run(max(0.0, t0 + q * 1e-11 - s.getv("t")))
+        table.append(s.getv("<m>"))
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - ts
+    steps = s.getv("step") - step0
+    sim_ns = (s.getv("t") - t0) * 1e9
+    m_end = s.getarray("m").astype(np.float64)
+    h_gpu = s.getarray("H_eff").astype(np.float64)
+    prob = O.Problem(n, cell, msat=cfg["msat"], aex=cfg["aex"], alpha=cfg["alpha"],
+                     bext=cfg["bext"])
+    h_orc = prob.heff(m_end)
+    h_err = float(np.abs(h_gpu - h_orc).max() / np.abs(h_orc).max())
+    s.close()
+
+    cpu = None
+    if not args.no_cpu:
+        mo = cfg["m"].astype(np.float64).copy()
+        prob.set("maxerr", cfg["maxerr"])
+        prob.step(mo)                  # warm-up
+        tc0, sc0 = prob.get("t"), prob.get("step")
+        tw0 = time.perf_counter()
+        while time.perf_counter() - tw0 < 10.0:
+            prob.step(mo)
+        wc = time.perf_counter() - tw0
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        cpu = {"value": (prob.get("t") - tc0) * 1e9 / wc, "unit": "simulated ns/s",
+               "cores": threads, "kind": "port",
+               "sample": "%d accepted FP64 RK12 steps of the oracle from the same state (%.1f s)"
+                         % (prob.get("step") - sc0, wc)}
+    return {
+        "metric": "H_eff evals/s + simulated ns/s at 256^2x64 FP32; surrogate vs exact, same GPU",
+        "value": sim_ns / wall, "unit": "simulated ns/s", "n_gpus": 1, "steps": int(steps),
+        "warmup": 1, "ms_per_step": 1e3 * wall / max(1, steps), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "film 128x32x1 cells of 3.90625x3.90625x3 nm, Ms 8e5, A 1.3e-11, "
+                               "alpha 0.02, seeded random m, 10 mT in-plane, demag + exchange, "
+                               "RK12 m_maxerror 1e-5, 1 ns with <m> every 10 ps",
+                   "steps_are": "accepted RK12 steps"},
+        "cpu_baseline": cpu,
+        "simulated_ns": sim_ns, "wall_s": wall, "us_per_step": 1e6 * wall / max(1, steps),
+        "final_<m>": table[-1], "<m>_every_100ps": table[9::10],
+        "final_H_eff_rel_err_vs_fp64_oracle": h_err,
+    }
+
+
 def run_surrogate_kernels(args):
     """The surrogate's kernels alone at the learned-correction configuration's
     size (film 256x256x4), for profiling them: `--steps` training steps (the
@@ -691,12 +761,14 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--rk-steps", type=int, default=1000)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=("bulk", "precision", "batch", "surrogate",
+    ap.add_argument("--workload", choices=("bulk", "film", "precision", "batch", "surrogate",
                                            "surrogate-kernels", "demagexch"),
                     default="bulk",
-                    help="bulk: 256x256x64 (the headline line); precision: 1024x1024x32 film "
-                         "with the FP64 Newell kernel and per-field errors against the FP64 "
-                         "oracle")
+                    help="bulk: 256x256x64 (the headline line); film: 128x32x1 RK12 for 1 ns "
+                         "beside the oracle's RK12; precision: 1024x1024x32 film with the FP64 "
+                         "Newell kernel and per-field errors against the FP64 oracle; batch: 64 "
+                         "uneven films; surrogate: training and surrogate-assisted runs; "
+                         "surrogate-kernels: the surrogate's kernels alone; demagexch")
     ap.add_argument("--train-samples", type=int, default=20000,
                     help="surrogate workload: training pairs from exact trajectories")
     ap.add_argument("--sim-time", type=float, default=1e-9,
@@ -727,6 +799,9 @@ def main():
         return
     if args.workload == "surrogate":
         print(json.dumps(run_surrogate(args)))
+        return
+    if args.workload == "film":
+        print(json.dumps(run_film(args)))
         return
     if args.workload == "surrogate-kernels":
         print(json.dumps(run_surrogate_kernels(args)))
