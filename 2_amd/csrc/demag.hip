@@ -12,6 +12,9 @@
 // For p0 == 1 (films) YF/Z/YI collapse into one kernel.
 // Replaces MaxwellPlan.update (maxwell_convolution.go:272-309): zeroing,
 // CopyPad3D, 3 cuFFT R2C, 9 cmadd passes, 3 cuFFT C2R and CopyUnPad3D.
+#include <atomic>
+#include <cstdlib>
+
 #include "demag_tiles.hpp"
 #include "reg_fft.hpp"
 
@@ -24,9 +27,15 @@ __global__ void __launch_bounds__(kBlock)
 xf_kernel(DemagGeom g, LineFFT<float> fx, int RB, const float *__restrict__ m,
           const float *__restrict__ msat_map, const float *__restrict__ tau0, float h,
           const float *__restrict__ norm_map, float *__restrict__ mpred,
-          float2 *__restrict__ xb)
+          float2 *__restrict__ xb, unsigned int *ho_dev, unsigned int *ho_host)
 {
     extern __shared__ float2 sm[];
+    if (ho_dev != nullptr && blockIdx.x == 0 && threadIdx.x == 0) {
+        // the previous launch's result word to the host, cleared for the next
+        const unsigned int e = __hip_atomic_load(ho_dev, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(ho_host, e, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        __hip_atomic_store(ho_dev, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
     xf_tile<PRED>(g, fx, RB, (long)blockIdx.x * RB, m, msat_map, tau0, h, norm_map, mpred, xb,
                   sm);
 }
@@ -133,7 +142,9 @@ static int y_and_z_passes(DemagPlanImpl *P, hipStream_t st)
 {
     const DemagGeom &g = P->g;
     const int ntx_y = div_up(g.h2, P->tiles.w_y);
-    if (g.p0 == 1 && P->fast_y) {
+    // films: the register y pass up to length 256; beyond it holds too few
+    // workgroups (1x256x256: 17, 35.8 us against 24.4 for the generic tiles)
+    if (g.p0 == 1 && P->fast_y && g.p1 <= kFastY2dMaxLength) {
         MMX_TRY(fast_y2d(P, st));
         mark(P, st, PASS_Y2D);
         return MMX_OK;
@@ -188,6 +199,20 @@ static int y_and_z_passes(DemagPlanImpl *P, hipStream_t st)
     return MMX_OK;
 }
 
+static std::atomic<int> g_fast_x_min{kFastXMinBlocks};
+
+int fast_x_min_blocks() { return g_fast_x_min.load(std::memory_order_relaxed); }
+
+int set_fast_x_min_blocks(int n)
+{
+    return g_fast_x_min.exchange(n, std::memory_order_relaxed);
+}
+
+bool fast_xf_used(const DemagPlanImpl *P)
+{
+    return P->fast_x && fast_x_blocks(P->g) >= fast_x_min_blocks();
+}
+
 int demag_forward(DemagPlanImpl *P, const float *m, const float *msat_map, hipStream_t st)
 {
     const DemagGeom &g = P->g;
@@ -195,12 +220,12 @@ int demag_forward(DemagPlanImpl *P, const float *m, const float *msat_map, hipSt
     allow_lds(xf_kernel<0>, b);
     const int nb = div_up(g.nrows, P->tiles.rows_x);
     mark(P, st, PASS_START);
-    if (P->fast_x) {
+    if (fast_xf_used(P)) {
         MMX_TRY(fast_xf(P, 0, m, msat_map, nullptr, 0.0f, nullptr, nullptr, st));
     } else {
-        MMX_REQUIRE(!P->handoff_dev, MMX_ERR_ARG, "demag_forward: hand-off needs the fast x pass");
         xf_kernel<0><<<dim3(nb), dim3(kBlock), b, st>>>(g, P->fx, P->tiles.rows_x, m, msat_map,
-                                                         nullptr, 0.0f, nullptr, nullptr, P->xbuf);
+                                                         nullptr, 0.0f, nullptr, nullptr, P->xbuf,
+                                                         P->handoff_dev, P->handoff_host);
         MMX_CHECK_LAUNCH();
     }
     if (P->after_x) MMX_HIP(hipEventRecord(P->after_x, st));
@@ -217,11 +242,12 @@ int demag_forward_pred(DemagPlanImpl *P, const float *m0, const float *tau0, flo
     const int nb = div_up(g.nrows, P->tiles.rows_x);
     // the demag source is M = m * Msat_map, norm map = Msat map
     mark(P, st, PASS_START);
-    if (P->fast_x) {
+    if (fast_xf_used(P)) {
         MMX_TRY(fast_xf(P, 1, m0, norm_map, tau0, h, norm_map, mpred, st));
     } else {
         xf_kernel<1><<<dim3(nb), dim3(kBlock), b, st>>>(g, P->fx, P->tiles.rows_x, m0, norm_map,
-                                                         tau0, h, norm_map, mpred, P->xbuf);
+                                                         tau0, h, norm_map, mpred, P->xbuf, nullptr,
+                                                         nullptr);
         MMX_CHECK_LAUNCH();
     }
     mark(P, st, PASS_XF);
@@ -236,7 +262,7 @@ int demag_inverse(DemagPlanImpl *P, int mode, const FieldParams &fp, const Epilo
     const int nb = div_up(g.nrows, P->tiles.rows_x);
     const int RB = P->tiles.rows_x;
     const int pass_xi = (g.p0 == 1) ? 2 : 4;
-    if (P->fast_x) {
+    if (P->fast_x && fast_x_blocks(g) >= fast_x_min_blocks()) {
         MMX_TRY(fast_xi(P, mode, fp, io, st));
         mark(P, st, pass_xi);
         return MMX_OK;

@@ -106,6 +106,18 @@ int fast_xf(const DemagPlanImpl *P, int pred, const float *m, const float *msat_
     }
 }
 
+// Workgroups of the register-FFT x pass (RowTile rows per workgroup).
+int fast_x_blocks(const DemagGeom &g)
+{
+    switch (g.Lx) {
+#define CASE(L)                                                                              \
+    case L: return div_up((int)g.nrows, RowTile<L>::R);
+        MMX_SHORT_LENGTHS(CASE)
+#undef CASE
+    default: return 0;
+    }
+}
+
 bool fast_xi_applies(const FieldParams &fp)
 {
     return !fp.msat_map && !fp.aex_map && !fp.alpha_map && !fp.ku_map && !fp.zl && !fp.htherm &&

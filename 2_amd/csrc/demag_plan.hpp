@@ -96,10 +96,10 @@ struct DemagPlanImpl {
     hipEvent_t *events = nullptr; // when set: recorded before/after every launch
     int *event_pass = nullptr;    // pass id of the launch that each event ends
     int nevent = 0;
-    // Set by the caller around one demag_forward on the register-FFT path:
-    // the x pass's first thread moves the word *handoff_dev (left by the
-    // previous launch on the stream) into the pinned host word *handoff_host
-    // and clears it, and after_x is recorded once the x pass has been queued.
+    // Set by the caller around one demag_forward: the x pass's first thread
+    // moves the word *handoff_dev (left by the previous launch on the stream)
+    // into the pinned host word *handoff_host and clears it, and after_x is
+    // recorded once the x pass has been queued.
     unsigned int *handoff_dev = nullptr, *handoff_host = nullptr;
     hipEvent_t after_x = nullptr;
     // ConvPlan: the real-space kernel comes from the caller (6 host float32
@@ -144,6 +144,21 @@ bool fast_z_addressable(const DemagGeom &g);
 bool fast_x_addressable(const DemagGeom &g);
 int fast_yi(const DemagPlanImpl *P, hipStream_t st);
 int fast_y2d(const DemagPlanImpl *P, hipStream_t st);
+int fast_x_blocks(const DemagGeom &g);
+// The register-FFT x passes run when they have at least this many
+// workgroups (one per CU); below it a handful of workgroups each carry a long
+// serial chain and the generic LDS kernels, spread over more workgroups, are
+// faster (128x32x1 film: xf 8.3 against 12.5 us, xi 9.8 against 17.6 us;
+// 4x256x256, 64 workgroups: xf 13.1 against 16.3, xi 14.6 against 20.2; at
+// 16x256x256, 256 workgroups, the register passes win: xf 17.2 against 30.5).
+constexpr int kFastXMinBlocks = 256;
+// the value in force (mmx_set_fast_x_min_blocks)
+int fast_x_min_blocks();
+int set_fast_x_min_blocks(int n); // returns the previous value
+// Films run the register y pass up to this length.
+constexpr int kFastY2dMaxLength = 256;
+// Whether demag_forward runs the register-FFT x pass for this plan.
+bool fast_xf_used(const DemagPlanImpl *P);
 int fast_xi(const DemagPlanImpl *P, int mode, const FieldParams &fp, const EpilogueIO &io,
             hipStream_t st);
 // the fast C2R pass's pair epilogue covers uniform parameters without the

@@ -247,6 +247,14 @@ int mmx_demag_update_b(mmx_demag_plan *plan, float *bx, float *by, float *bz, co
     return demag_inverse(P, EPI_B, fp, io, st);
 }
 
+int mmx_set_fast_x_min_blocks(int nblocks, int *previous)
+{
+    MMX_REQUIRE(nblocks >= 0, MMX_ERR_ARG, "mmx_set_fast_x_min_blocks: negative count %d", nblocks);
+    const int old = set_fast_x_min_blocks(nblocks);
+    if (previous) *previous = old;
+    return MMX_OK;
+}
+
 int mmx_demag_plan_info(const mmx_demag_plan *plan, int p[3], int ko[3], double *bad_over_good)
 {
     MMX_REQUIRE(plan, MMX_ERR_ARG, "mmx_demag_plan_info: NULL plan");

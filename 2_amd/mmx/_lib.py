@@ -44,6 +44,7 @@ _SIGS = {
     "mmx_demag_plan_destroy": (_i, [_vp]),
     "mmx_demag_update_b": (_i, [_vp, _fp, _fp, _fp, _fp, _fp, _fp, _fp, _f, _vp]),
     "mmx_demag_plan_info": (_i, [_vp, C.POINTER(_i), C.POINTER(_i), C.POINTER(_d)]),
+    "mmx_set_fast_x_min_blocks": (_i, [_i, C.POINTER(_i)]),
     "mmx_demag_plan_kernel": (_i, [_vp, _vp]),
     "mmx_demag_plan_real_kernel": (_i, [_vp, _vp]),
     "mmx_sim_create": (_i, [C.POINTER(_vp), _i, _i, _i, _d, _d, _d, _i, _i, _i]),

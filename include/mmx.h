@@ -204,6 +204,14 @@ int mmx_demag_update_b(mmx_demag_plan *plan, float *bx, float *by, float *bz,
                        const float *mx, const float *my, const float *mz,
                        const float *msat_map, float msat_mul, void *stream);
 
+/* Tuning of the demag x passes (no counterpart in the reference, whose FFT is
+ * cuFFT's choice): the register-FFT x passes run when they span at least
+ * `nblocks` workgroups, smaller grids take the generic LDS passes (faster
+ * there: fewer, longer serial chains otherwise).  Default 256; 0 runs the
+ * register passes wherever their lengths allow.  Process-wide; *previous (if
+ * not NULL) receives the value it replaces. */
+int mmx_set_fast_x_min_blocks(int nblocks, int *previous);
+
 /* Plan geometry: padded size p[3]; octant kernel size ko[3] = (p0/2+1,
  * p1/2+1, p2/2+1); realness ratio max|Im|/max|Re| of the FFT'd kernel
  * (the reference panics above 1e-5, maxwell_convolution.go:608-610). */

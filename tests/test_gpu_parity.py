@@ -30,6 +30,20 @@ def torch_gpu():
     return torch
 
 
+@pytest.fixture(scope="module", autouse=True)
+def register_x_passes_everywhere():
+    """The register-FFT x passes at every grid size (by default grids of
+    fewer than 256 x-pass workgroups take the generic passes): these tests pin
+    those kernels at small sizes; the default choice is covered by
+    tests/test_gpu_small_grids.py and the other modules."""
+    import ctypes as C
+    import mmx
+    prev = C.c_int()
+    mmx.check(mmx.lib().mmx_set_fast_x_min_blocks(0, C.byref(prev)))
+    yield
+    mmx.check(mmx.lib().mmx_set_fast_x_min_blocks(prev.value, None))
+
+
 def relerr(a, b):
     return float(np.abs(np.asarray(a, np.float64) - b).max() / max(np.abs(b).max(), 1e-300))
 
