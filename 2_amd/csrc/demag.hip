@@ -142,10 +142,10 @@ static int y_and_z_passes(DemagPlanImpl *P, hipStream_t st)
 {
     const DemagGeom &g = P->g;
     const int ntx_y = div_up(g.h2, P->tiles.w_y);
-    // films: the register y pass up to length 256; beyond it holds too few
-    // workgroups (1x256x256: 17, 35.8 us against 24.4 for the generic tiles)
-    if (g.p0 == 1 && P->fast_y && g.p1 <= kFastY2dMaxLength) {
-        MMX_TRY(fast_y2d(P, st));
+    // films: the y transform on the z pass's kernel (three component groups
+    // per column tile) wherever the length has a register FFT
+    if (g.p0 == 1 && fast_y2d_as_z_applies(g)) {
+        MMX_TRY(fast_y2d_as_z(P, st));
         mark(P, st, PASS_Y2D);
         return MMX_OK;
     }

@@ -143,8 +143,9 @@ int fast_z(const DemagPlanImpl *P, hipStream_t st);
 bool fast_z_addressable(const DemagGeom &g);
 bool fast_x_addressable(const DemagGeom &g);
 int fast_yi(const DemagPlanImpl *P, hipStream_t st);
-int fast_y2d(const DemagPlanImpl *P, hipStream_t st);
 int fast_x_blocks(const DemagGeom &g);
+bool fast_y2d_as_z_applies(const DemagGeom &g);
+int fast_y2d_as_z(const DemagPlanImpl *P, hipStream_t st);
 // The register-FFT x passes run when they have at least this many
 // workgroups (one per CU); below it a handful of workgroups each carry a long
 // serial chain and the generic LDS kernels, spread over more workgroups, are
@@ -155,8 +156,6 @@ constexpr int kFastXMinBlocks = 256;
 // the value in force (mmx_set_fast_x_min_blocks)
 int fast_x_min_blocks();
 int set_fast_x_min_blocks(int n); // returns the previous value
-// Films run the register y pass up to this length.
-constexpr int kFastY2dMaxLength = 256;
 // Whether demag_forward runs the register-FFT x pass for this plan.
 bool fast_xf_used(const DemagPlanImpl *P);
 int fast_xi(const DemagPlanImpl *P, int mode, const FieldParams &fp, const EpilogueIO &io,

@@ -1,7 +1,8 @@
 // fast_kernels.hpp -- the demag pass kernels for power-of-two transform
 // lengths (register four-step FFTs), shared by demag_fast.hip and xi_gen.hip.
 //
-// Same five passes as demag.hip (xf, yf, z, yi, xi; films: xf, y2d, xi) with
+// Same five passes as demag.hip (xf, yf, z, yi, xi; films: xf, the y
+// transform with the kernel multiply on z_fast, xi) with
 // the FFT held in registers (four-step split, reg_fft.hpp): each pass reads its
 // input once and writes its output once, with one LDS transpose per transform
 // direction.  Strided passes work on tiles of W spectrum columns (W >= 16, so
@@ -376,7 +377,7 @@ __device__ __forceinline__ int mirror_order(int i, int p)
 template <int L, bool FULL>
 __global__ void __launch_bounds__(ZTile<L>::NT)
 z_fast(DemagGeom g, int ntx, const float2 *__restrict__ tw, const float *__restrict__ K,
-       float2 *__restrict__ yb)
+       float2 *__restrict__ yb, int film_y)
 {
     using Z = ZTile<L>;
     constexpr int NQ = FULL ? Z::E : Z::E / 2; // planes t + T q held, q < NQ
@@ -442,8 +443,13 @@ z_fast(DemagGeom g, int ntx, const float2 *__restrict__ tw, const float *__restr
         kern_apply(k, sm[(0 * L + r1) * Z::W + w2], sm[(1 * L + r1) * Z::W + w2],
                    sm[(2 * L + r1) * Z::W + w2]);
         if (kzp > 0 && kzp < L / 2) {
-            Kern6 km = k; // K at p0 - kz: odd in z for XZ and XY
-            km.xz = -km.xz;
+            // K at L - k: odd in the transformed axis, which holds XZ and XY
+            // (internal axis 0) or, for a film's y pass (film_y), YZ and XY
+            Kern6 km = k;
+            if (film_y)
+                km.yz = -km.yz;
+            else
+                km.xz = -km.xz;
             km.xy = -km.xy;
             const int r2 = zrow<L>(L - kzp);
             kern_apply(km, sm[(0 * L + r2) * Z::W + w2], sm[(1 * L + r2) * Z::W + w2],
@@ -484,56 +490,6 @@ z_fast(DemagGeom g, int ntx, const float2 *__restrict__ tw, const float *__restr
     if (ok) {
 #pragma unroll
         for (int q = 0; q < NQ; q++) st_off<float2>(cbase + (size_t)q * Z::T * istride, voff, v[q]);
-    }
-}
-
-// films (p0 == 1): y transform, kernel multiply and inverse y in one kernel
-// (FULL: periodic y, n1 == L; otherwise rows n1 .. L-1 are zero padding)
-template <int L, bool FULL>
-__global__ void __launch_bounds__(ColTile<L>::NT)
-y2d_fast(DemagGeom g, int ntx, const float2 *__restrict__ tw, const float *__restrict__ K,
-         float2 *__restrict__ xb)
-{
-    using C = ColTile<L>;
-    constexpr int NQ = FULL ? C::E : C::E / 2;
-    extern __shared__ float2 sm[];
-    float2 *sbuf = sm, *stw = sm + C::E * C::S;
-    const int tile = xcd_tile(blockIdx.x, gridDim.x);
-    const int kx0 = tile * C::W;
-    const int w = threadIdx.x % C::W, t = threadIdx.x / C::W;
-    for (int q = threadIdx.x; q < L; q += C::NT) stw[q] = tw[q];
-    const bool ok = kx0 + w < g.h2;
-    const int col = ok ? kx0 + w : g.h2 - 1;
-    const size_t cstride = (size_t)g.nrows * g.sx;
-    const unsigned off = (unsigned)(((size_t)t * g.sx + col) * sizeof(float2));
-    float2 u[3][C::G][C::T];
-    __syncthreads();
-#pragma unroll
-    for (int c = 0; c < 3; c++) {
-        float2 v[C::E];
-#pragma unroll
-        for (int q = 0; q < NQ; q++)
-            v[q] = ld_off<float2>(xb + c * cstride + (size_t)q * C::T * g.sx, off);
-        col_forward<L, C::W, C::S, !FULL>(v, u[c], w, t, sbuf, stw);
-    }
-    if (ok) {
-#pragma unroll
-        for (int gg = 0; gg < C::G; gg++)
-#pragma unroll
-            for (int k2 = 0; k2 < C::T; k2++) {
-                const Kern6 k = kern_at(g, K, 0, t * C::G + gg + C::E * k2, kx0 + w);
-                kern_apply(k, u[0][gg][k2], u[1][gg][k2], u[2][gg][k2]);
-            }
-    }
-#pragma unroll
-    for (int c = 0; c < 3; c++) {
-        float2 v[C::E];
-        col_inverse<L, C::W, C::S>(u[c], v, w, t, sbuf, stw);
-        if (ok) {
-#pragma unroll
-            for (int q = 0; q < NQ; q++)
-                st_off<float2>(xb + c * cstride + (size_t)q * C::T * g.sx, off, v[q]);
-        }
     }
 }
 
@@ -915,7 +871,8 @@ int xf_len(const DemagPlanImpl *P, int pred, const float *m, const float *msat_m
 template <int L> int yf_len(const DemagPlanImpl *P, hipStream_t st);
 template <int L> int yi_len(const DemagPlanImpl *P, hipStream_t st);
 template <int L> int z_len(const DemagPlanImpl *P, hipStream_t st);
-template <int L> int y2d_len(const DemagPlanImpl *P, hipStream_t st);
+template <int L>
+int z_len_geom(const DemagGeom &g, const float2 *tw, const float *K, float2 *buf, hipStream_t st);
 template <int L, bool GEN>
 int xi_len(const DemagPlanImpl *P, int mode, const FieldParams &fp, const EpilogueIO &io,
            hipStream_t st);

@@ -83,19 +83,22 @@ template <int L> int z_len(const DemagPlanImpl *P, hipStream_t st)
     const int ntx = div_up(g.h2, Z::W);
     auto kern = (g.n0 == g.p0) ? z_fast<L, true> : z_fast<L, false>;
     allow_lds_fast(kern, Z::LDS);
-    kern<<<dim3(ntx * g.p1), dim3(Z::NT), Z::LDS, st>>>(g, ntx, P->fz.tw, P->kern, P->ybuf);
+    kern<<<dim3(ntx * g.p1), dim3(Z::NT), Z::LDS, st>>>(g, ntx, P->fz.tw, P->kern, P->ybuf, 0);
     MMX_CHECK_LAUNCH();
     return MMX_OK;
 }
 
-template <int L> int y2d_len(const DemagPlanImpl *P, hipStream_t st)
+// The z pass's kernel on a caller's geometry: a film's y transform with the
+// kernel multiply, the film's rows taking the place of the z planes
+// (fast_y2d_as_z in demag_fast.hip).
+template <int L>
+int z_len_geom(const DemagGeom &g, const float2 *tw, const float *K, float2 *buf, hipStream_t st)
 {
-    const DemagGeom &g = P->g;
-    using C = ColTile<L>;
-    const int ntx = div_up(g.h2, C::W);
-    auto kern = (g.n1 == g.p1) ? y2d_fast<L, true> : y2d_fast<L, false>;
-    allow_lds_fast(kern, C::LDS);
-    kern<<<dim3(ntx), dim3(C::NT), C::LDS, st>>>(g, ntx, P->fy.tw, P->kern, P->xbuf);
+    using Z = ZTile<L>;
+    const int ntx = div_up(g.h2, Z::W);
+    auto kern = (g.n0 == g.p0) ? z_fast<L, true> : z_fast<L, false>;
+    allow_lds_fast(kern, Z::LDS);
+    kern<<<dim3(ntx * g.p1), dim3(Z::NT), Z::LDS, st>>>(g, ntx, tw, K, buf, 1);
     MMX_CHECK_LAUNCH();
     return MMX_OK;
 }
@@ -134,8 +137,10 @@ int xi_len(const DemagPlanImpl *P, int mode, const FieldParams &fp, const Epilog
 #define MMX_INST_Y(L)                                                                        \
     template int yf_len<L>(const DemagPlanImpl *, hipStream_t);                              \
     template int yi_len<L>(const DemagPlanImpl *, hipStream_t);
-#define MMX_INST_Z(L) template int z_len<L>(const DemagPlanImpl *, hipStream_t);
-#define MMX_INST_Y2D(L) template int y2d_len<L>(const DemagPlanImpl *, hipStream_t);
+#define MMX_INST_Z(L)                                                                        \
+    template int z_len<L>(const DemagPlanImpl *, hipStream_t);                               \
+    template int z_len_geom<L>(const DemagGeom &, const float2 *, const float *, float2 *,   \
+                               hipStream_t);
 #define MMX_INST_XI(L, GEN)                                                                  \
     template int xi_len<L, GEN>(const DemagPlanImpl *, int, const FieldParams &,             \
                                 const EpilogueIO &, hipStream_t);

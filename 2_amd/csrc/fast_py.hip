@@ -1,4 +1,4 @@
-// fast_py.hip -- the y-pass launchers (3-D: yf, yi; films: y2d) of the
+// fast_py.hip -- the 3-D y-pass launchers (yf, yi) of the
 // power-of-two lengths (fast_launch.hpp), in a file of its own so that it
 // compiles in parallel with the other instantiations.
 #include "fast_launch.hpp"
@@ -6,6 +6,5 @@
 namespace mmx {
 
 MMX_POW2_LENGTHS(MMX_INST_Y)
-MMX_POW2_LENGTHS(MMX_INST_Y2D)
 
 } // namespace mmx

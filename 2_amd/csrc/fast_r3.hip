@@ -9,7 +9,6 @@ namespace mmx {
 #define MMX_INST_XI_BOTH(L) MMX_INST_XI(L, false) MMX_INST_XI(L, true)
 MMX_R3_SHORT(MMX_INST_XF)
 MMX_R3_SHORT(MMX_INST_XI_BOTH)
-MMX_R3_SHORT(MMX_INST_Y2D)
 MMX_R3_SHORT(MMX_INST_Z)
 MMX_R3_SHORT(MMX_INST_Y)
 MMX_R3_LONG(MMX_INST_Y)
