@@ -145,8 +145,13 @@ int fast_x_blocks(const DemagGeom &g)
 
 bool fast_xi_applies(const FieldParams &fp)
 {
-    return !fp.msat_map && !fp.aex_map && !fp.alpha_map && !fp.ku_map && !fp.zl && !fp.htherm &&
-           !fp.bext_map && !fp.u_map && !fp.hdex;
+    return !fp.msat_map && !fp.aex_map && fast_xi_maps_applies(fp);
+}
+
+bool fast_xi_maps_applies(const FieldParams &fp)
+{
+    return !fp.alpha_map && !fp.ku_map && !fp.zl && !fp.htherm && !fp.bext_map && !fp.u_map &&
+           !fp.hdex;
 }
 
 int fast_xi(const DemagPlanImpl *P, int mode, const FieldParams &fp, const EpilogueIO &io,
@@ -154,10 +159,15 @@ int fast_xi(const DemagPlanImpl *P, int mode, const FieldParams &fp, const Epilo
 {
     const DemagGeom &g = P->g;
     MMX_REQUIRE(fast_x_addressable(g), MMX_ERR_SIZE, "fast_xi: unsupported geometry");
-    const bool gen = mode != EPI_B && !fast_xi_applies(fp);
+    const int var = (mode == EPI_B || fast_xi_applies(fp)) ? XI_UNIFORM
+                    : fast_xi_maps_applies(fp)                ? XI_MAPS
+                                                              : XI_GENERAL;
     switch (g.Lx) {
 #define CASE(L)                                                                              \
-    case L: return gen ? xi_len<L, true>(P, mode, fp, io, st) : xi_len<L, false>(P, mode, fp, io, st);
+    case L:                                                                                  \
+        return var == XI_UNIFORM ? xi_len<L, XI_UNIFORM>(P, mode, fp, io, st)                \
+               : var == XI_MAPS  ? xi_len<L, XI_MAPS>(P, mode, fp, io, st)                   \
+                                 : xi_len<L, XI_GENERAL>(P, mode, fp, io, st);
         MMX_SHORT_LENGTHS(CASE)
 #undef CASE
     default: set_error("fast_xi: length %d", g.Lx); return MMX_ERR_SIZE;

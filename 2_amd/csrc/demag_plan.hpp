@@ -160,12 +160,13 @@ int set_fast_x_min_blocks(int n); // returns the previous value
 bool fast_xf_used(const DemagPlanImpl *P);
 int fast_xi(const DemagPlanImpl *P, int mode, const FieldParams &fp, const EpilogueIO &io,
             hipStream_t st);
-// the fast C2R pass's pair epilogue covers uniform parameters without the
-// spin-transfer term; otherwise fast_xi runs the general per-cell epilogue
-// (fast_xi_general, xi_gen.hip)
+// The fast C2R pass's two-cell epilogue covers uniform parameters
+// (fast_xi_applies) and Msat / Aex maps as the only per-cell terms
+// (fast_xi_maps_applies: no other map or mask, no spin-transfer term, no
+// thermal or demag-exchange field); otherwise fast_xi runs the general
+// per-cell epilogue (xi_gen.hip).
 bool fast_xi_applies(const FieldParams &fp);
-int fast_xi_general(const DemagPlanImpl *P, int mode, const FieldParams &fp, const EpilogueIO &io,
-                    hipStream_t st);
+bool fast_xi_maps_applies(const FieldParams &fp);
 
 } // namespace mmx
 

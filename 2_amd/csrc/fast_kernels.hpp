@@ -663,9 +663,10 @@ xf_fast(DemagGeom g, const float2 *__restrict__ tw, const float2 *__restrict__ r
 }
 
 // Two neighbouring cells (k0 even, k0 + 1 < n2) of one row at once, for
-// uniform material parameters (no maps) and no spin-transfer term: the same
-// per-cell arithmetic as cell_epilogue (exchange6 order dim 0, dim 2, dim 1;
-// SumUpdater order), with 8-byte loads and stores of the cell pair.
+// uniform material parameters (MAPS false) or Msat / Aex maps only (MAPS
+// true), and no spin-transfer term: the same per-cell arithmetic as
+// cell_epilogue (exchange6 order dim 0, dim 2, dim 1; SumUpdater order), with
+// 8-byte loads and stores of the cell pair.
 
 __device__ __forceinline__ int nb_lo(int pos, int len, int wrap)
 {
@@ -680,7 +681,22 @@ __device__ __forceinline__ int nb_hi(int pos, int len, int wrap)
     return min(hi, len - 1);
 }
 
-template <int MODE>
+// Aex/Msat of the cell pair at byte offset b (div_zero(mask_unity(aex),
+// mask_unity(msat)) of exchange6_at), and of one cell
+__device__ __forceinline__ float2 lex_pair(const FieldParams &fp, unsigned b)
+{
+    const float2 a = fp.aex_map ? ld_off<float2>(fp.aex_map, b) : float2{1.0f, 1.0f};
+    const float2 s = fp.msat_map ? ld_off<float2>(fp.msat_map, b) : float2{1.0f, 1.0f};
+    return float2{div_zero(a.x, s.x), div_zero(a.y, s.y)};
+}
+__device__ __forceinline__ float lex_one(const FieldParams &fp, unsigned b)
+{
+    const float a = fp.aex_map ? ld_off<float>(fp.aex_map, b) : 1.0f;
+    const float s = fp.msat_map ? ld_off<float>(fp.msat_map, b) : 1.0f;
+    return div_zero(a, s);
+}
+
+template <int MODE, bool MAPS = false>
 __device__ __forceinline__ void pair_epilogue(const DemagGeom &g, const FieldParams &fp,
                                               const EpilogueIO &io, unsigned I0, int i, int j,
                                               int k0, const float hd0[3], const float hd1[3],
@@ -712,29 +728,69 @@ __device__ __forceinline__ void pair_epilogue(const DemagGeom &g, const FieldPar
     const unsigned o2h = b0 + 4u * (unsigned)(nb_hi(k0 + 1, g.n2, fp.pbc[2]) - k0); // right of 1
     const float p0 = fp.ex_pre * fp.ex_c[0], p1 = fp.ex_pre * fp.ex_c[1],
                 p2 = fp.ex_pre * fp.ex_c[2];
+    if constexpr (MAPS) {
+        // exchange weights: harmonic mean of the pair's and each neighbour's
+        // Aex/Msat (w<dim><lo|hi><cell>)
+        const float2 lc = lex_pair(fp, b0);
+        const float2 lzl = lex_pair(fp, o0l), lzh = lex_pair(fp, o0h);
+        const float2 lyl = lex_pair(fp, o1l), lyh = lex_pair(fp, o1h);
+        const float lxl = lex_one(fp, o2l), lxh = lex_one(fp, o2h);
+        const float w0l0 = avg_geom_zero(lc.x, lzl.x), w0h0 = avg_geom_zero(lc.x, lzh.x);
+        const float w0l1 = avg_geom_zero(lc.y, lzl.y), w0h1 = avg_geom_zero(lc.y, lzh.y);
+        const float w2l0 = avg_geom_zero(lc.x, lxl), w2h0 = avg_geom_zero(lc.x, lc.y);
+        const float w2l1 = avg_geom_zero(lc.y, lc.x), w2h1 = avg_geom_zero(lc.y, lxh);
+        const float w1l0 = avg_geom_zero(lc.x, lyl.x), w1h0 = avg_geom_zero(lc.x, lyh.x);
+        const float w1l1 = avg_geom_zero(lc.y, lyl.y), w1h1 = avg_geom_zero(lc.y, lyh.y);
 #pragma unroll
-    for (int c = 0; c < 3; c++) {
-        const float *mc = m + c * N;
-        const float2 zl = ld_off<float2>(mc, o0l), zh = ld_off<float2>(mc, o0h);
-        const float2 yl = ld_off<float2>(mc, o1l), yh = ld_off<float2>(mc, o1h);
-        const float xl = ld_off<float>(mc, o2l), xh = ld_off<float>(mc, o2h);
-        float e0 = p0 * ((zl.x - m0[c]) + (zh.x - m0[c]));
-        float e1 = p0 * ((zl.y - m1[c]) + (zh.y - m1[c]));
-        e0 += p2 * ((xl - m0[c]) + (m1[c] - m0[c]));
-        e1 += p2 * ((m0[c] - m1[c]) + (xh - m1[c]));
-        e0 += p1 * ((yl.x - m0[c]) + (yh.x - m0[c]));
-        e1 += p1 * ((yl.y - m1[c]) + (yh.y - m1[c]));
-        H0[c] = hd0[c] + e0 + fp.hext[c];
-        H1[c] = hd1[c] + e1 + fp.hext[c];
+        for (int c = 0; c < 3; c++) {
+            const float *mc = m + c * N;
+            const float2 zl = ld_off<float2>(mc, o0l), zh = ld_off<float2>(mc, o0h);
+            const float2 yl = ld_off<float2>(mc, o1l), yh = ld_off<float2>(mc, o1h);
+            const float xl = ld_off<float>(mc, o2l), xh = ld_off<float>(mc, o2h);
+            float e0 = p0 * (w0l0 * (zl.x - m0[c]) + w0h0 * (zh.x - m0[c]));
+            float e1 = p0 * (w0l1 * (zl.y - m1[c]) + w0h1 * (zh.y - m1[c]));
+            e0 += p2 * (w2l0 * (xl - m0[c]) + w2h0 * (m1[c] - m0[c]));
+            e1 += p2 * (w2l1 * (m0[c] - m1[c]) + w2h1 * (xh - m1[c]));
+            e0 += p1 * (w1l0 * (yl.x - m0[c]) + w1h0 * (yh.x - m0[c]));
+            e1 += p1 * (w1l1 * (yl.y - m1[c]) + w1h1 * (yh.y - m1[c]));
+            H0[c] = hd0[c] + e0 + fp.hext[c];
+            H1[c] = hd1[c] + e1 + fp.hext[c];
+        }
+    } else {
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const float *mc = m + c * N;
+            const float2 zl = ld_off<float2>(mc, o0l), zh = ld_off<float2>(mc, o0h);
+            const float2 yl = ld_off<float2>(mc, o1l), yh = ld_off<float2>(mc, o1h);
+            const float xl = ld_off<float>(mc, o2l), xh = ld_off<float>(mc, o2h);
+            float e0 = p0 * ((zl.x - m0[c]) + (zh.x - m0[c]));
+            float e1 = p0 * ((zl.y - m1[c]) + (zh.y - m1[c]));
+            e0 += p2 * ((xl - m0[c]) + (m1[c] - m0[c]));
+            e1 += p2 * ((m0[c] - m1[c]) + (xh - m1[c]));
+            e0 += p1 * ((yl.x - m0[c]) + (yh.x - m0[c]));
+            e1 += p1 * ((yl.y - m1[c]) + (yh.y - m1[c]));
+            H0[c] = hd0[c] + e0 + fp.hext[c];
+            H1[c] = hd1[c] + e1 + fp.hext[c];
+        }
+    }
+    // Msat of the pair (a zero Msat reads as 1 in the anisotropy constant and
+    // masks the renormalised cell)
+    float2 ms = float2{1.0f, 1.0f};
+    if constexpr (MAPS) {
+        if (fp.msat_map) ms = ld_off<float2>(fp.msat_map, b0);
     }
     if (fp.anis) {
-        const float k = fp.ku2;
+        float k = fp.ku2, k1 = fp.ku2;
+        if constexpr (MAPS) {
+            k = fp.ku2 / (ms.x == 0.0f ? 1.0f : ms.x);
+            k1 = fp.ku2 / (ms.y == 0.0f ? 1.0f : ms.y);
+        }
         const float mu0 = m0[0] * fp.u[0] + m0[1] * fp.u[1] + m0[2] * fp.u[2];
         const float mu1 = m1[0] * fp.u[0] + m1[1] * fp.u[1] + m1[2] * fp.u[2];
 #pragma unroll
         for (int c = 0; c < 3; c++) {
             H0[c] += k * mu0 * fp.u[c];
-            H1[c] += k * mu1 * fp.u[c];
+            H1[c] += k1 * mu1 * fp.u[c];
         }
     }
     if (io.hout) {
@@ -763,8 +819,9 @@ __device__ __forceinline__ void pair_epilogue(const DemagGeom &g, const FieldPar
         errmax = fmaxf(errmax, (d != d) ? __builtin_inff() : d);
         const float l0 = sqrtf(a0[0] * a0[0] + a0[1] * a0[1] + a0[2] * a0[2]);
         const float l1 = sqrtf(a1[0] * a1[0] + a1[1] * a1[1] + a1[2] * a1[2]);
-        const float q0 = (l0 != 0.0f) ? 1.0f / l0 : 0.0f;
-        const float q1 = (l1 != 0.0f) ? 1.0f / l1 : 0.0f;
+        const float s0 = (ms.x == 0.0f) ? 0.0f : 1.0f, s1 = (ms.y == 0.0f) ? 0.0f : 1.0f;
+        const float q0 = (l0 != 0.0f) ? s0 / l0 : 0.0f;
+        const float q1 = (l1 != 0.0f) ? s1 / l1 : 0.0f;
 #pragma unroll
         for (int c = 0; c < 3; c++)
             st_off<float2>(io.mnew + c * N, b0, float2{a0[c] * q0, a1[c] * q1});
@@ -772,11 +829,27 @@ __device__ __forceinline__ void pair_epilogue(const DemagGeom &g, const FieldPar
 }
 
 // xi: C2R along x (imaginary parts of DC and Nyquist ignored), keep k < n2,
-// then the fused field / torque / RK epilogue
-// GEN: the general per-cell epilogue (cell_epilogue: maps, masks, Zhang-Li,
-// thermal field) instead of the pair epilogue for uniform parameters.
-template <int LX, int MODE, bool FULL, bool GEN>
+// then the fused field / torque / RK epilogue, one of three variants:
+// XI_UNIFORM the pair epilogue for uniform parameters, XI_MAPS the pair
+// epilogue with Msat / Aex maps, XI_GENERAL the per-cell epilogue
+// (cell_epilogue: every map and mask, Zhang-Li, the thermal field).  Each is
+// its own kernel: the general epilogue's register demand would otherwise set
+// the occupancy of the pair variants.
+enum XiVariant { XI_UNIFORM = 0, XI_MAPS = 1, XI_GENERAL = 2 };
+
+// Minimum waves per SIMD the xi kernel is built for.  The map and general
+// epilogues of the power-of-two lengths up to 256 take all 256 VGPRs (one
+// wave per SIMD) when left alone; bounded to two waves they spill a few
+// dozen registers and still run faster (256-point rows, Msat map: 146 ->
+// 103 us).  Longer rows, and the other lengths, spill hundreds at two waves.
+template <int LX, int VAR> constexpr int xi_min_waves()
+{
+    return (VAR != XI_UNIFORM && LX <= 256 && (LX & (LX - 1)) == 0) ? 2 : 1;
+}
+
+template <int LX, int MODE, bool FULL, int VAR>
 __global__ void __launch_bounds__(RowTile<LX>::NT)
+    __attribute__((amdgpu_waves_per_eu(xi_min_waves<LX, VAR>())))
 xi_fast(DemagGeom g, const float2 *__restrict__ tw, const float2 *__restrict__ rtw,
         const float2 *__restrict__ xb, FieldParams fp, EpilogueIO io)
 {
@@ -825,8 +898,7 @@ xi_fast(DemagGeom g, const float2 *__restrict__ tw, const float2 *__restrict__ r
 
     float errmax = 0.0f;
     // cell pairs (n2 is even on this path: n2 = LX padded to 2 LX, or periodic
-    // with n2 = 2 LX); without GEN only uniform parameters without the
-    // spin-transfer term (fast_xi_applies), with 8-byte I/O
+    // with n2 = 2 LX), with 8-byte I/O in the pair variants
     if (ok) {
         const int i = row / g.n1, j = row - i * g.n1;
         // the pairs' loads are left to the scheduler: a scheduling barrier per
@@ -840,13 +912,13 @@ xi_fast(DemagGeom g, const float2 *__restrict__ tw, const float2 *__restrict__ r
                 hd0[c] = fp.demag_scale * v[c][q].x;
                 hd1[c] = fp.demag_scale * v[c][q].y;
             }
-            if (GEN && MODE != EPI_B) {
+            if (VAR == XI_GENERAL && MODE != EPI_B) {
                 const long I0 = (long)row * g.n2 + k0;
                 cell_epilogue<MODE>(g, fp, io, I0, i, j, k0, hd0, errmax);
                 cell_epilogue<MODE>(g, fp, io, I0 + 1, i, j, k0 + 1, hd1, errmax);
             } else {
-                pair_epilogue<MODE>(g, fp, io, (unsigned)(row * g.n2 + k0), i, j, k0, hd0, hd1,
-                                    errmax);
+                pair_epilogue<MODE, VAR == XI_MAPS>(g, fp, io, (unsigned)(row * g.n2 + k0), i, j,
+                                                    k0, hd0, hd1, errmax);
             }
         }
     }
@@ -873,7 +945,7 @@ template <int L> int yi_len(const DemagPlanImpl *P, hipStream_t st);
 template <int L> int z_len(const DemagPlanImpl *P, hipStream_t st);
 template <int L>
 int z_len_geom(const DemagGeom &g, const float2 *tw, const float *K, float2 *buf, hipStream_t st);
-template <int L, bool GEN>
+template <int L, int VAR>
 int xi_len(const DemagPlanImpl *P, int mode, const FieldParams &fp, const EpilogueIO &io,
            hipStream_t st);
 

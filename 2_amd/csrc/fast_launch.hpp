@@ -103,28 +103,28 @@ int z_len_geom(const DemagGeom &g, const float2 *tw, const float *K, float2 *buf
     return MMX_OK;
 }
 
-// xi of length L in `mode` (GEN: the general per-cell epilogue; EPI_B always
-// runs the pair epilogue, which only stores the field)
-template <int L, int MODE, bool GEN>
+// xi of length L in `mode` with epilogue variant VAR (XiVariant; EPI_B
+// always runs the uniform one, which only stores the field)
+template <int L, int MODE, int VAR>
 static void xi_one(const DemagPlanImpl *P, const FieldParams &fp, const EpilogueIO &io,
                    hipStream_t st)
 {
     using C = RowTile<L>;
     const int nb = div_up(P->g.nrows, C::R);
-    auto kern = (P->g.n2 == 2 * L) ? xi_fast<L, MODE, true, GEN> : xi_fast<L, MODE, false, GEN>;
+    auto kern = (P->g.n2 == 2 * L) ? xi_fast<L, MODE, true, VAR> : xi_fast<L, MODE, false, VAR>;
     allow_lds_fast(kern, C::LDS);
     kern<<<dim3(nb), dim3(C::NT), C::LDS, st>>>(P->g, P->fx.tw, P->fx.rtw, P->xbuf, fp, io);
 }
 
-template <int L, bool GEN>
+template <int L, int VAR>
 int xi_len(const DemagPlanImpl *P, int mode, const FieldParams &fp, const EpilogueIO &io,
            hipStream_t st)
 {
     switch (mode) {
-    case EPI_B: xi_one<L, EPI_B, false>(P, fp, io, st); break;
-    case EPI_HEFF: xi_one<L, EPI_HEFF, GEN>(P, fp, io, st); break;
-    case EPI_TAU0: xi_one<L, EPI_TAU0, GEN>(P, fp, io, st); break;
-    default: xi_one<L, EPI_CORR, GEN>(P, fp, io, st); break;
+    case EPI_B: xi_one<L, EPI_B, XI_UNIFORM>(P, fp, io, st); break;
+    case EPI_HEFF: xi_one<L, EPI_HEFF, VAR>(P, fp, io, st); break;
+    case EPI_TAU0: xi_one<L, EPI_TAU0, VAR>(P, fp, io, st); break;
+    default: xi_one<L, EPI_CORR, VAR>(P, fp, io, st); break;
     }
     MMX_CHECK_LAUNCH();
     return MMX_OK;
@@ -141,8 +141,8 @@ int xi_len(const DemagPlanImpl *P, int mode, const FieldParams &fp, const Epilog
     template int z_len<L>(const DemagPlanImpl *, hipStream_t);                               \
     template int z_len_geom<L>(const DemagGeom &, const float2 *, const float *, float2 *,   \
                                hipStream_t);
-#define MMX_INST_XI(L, GEN)                                                                  \
-    template int xi_len<L, GEN>(const DemagPlanImpl *, int, const FieldParams &,             \
+#define MMX_INST_XI(L, VAR)                                                                  \
+    template int xi_len<L, VAR>(const DemagPlanImpl *, int, const FieldParams &,             \
                                 const EpilogueIO &, hipStream_t);
 
 } // namespace mmx

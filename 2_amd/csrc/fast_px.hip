@@ -6,7 +6,7 @@
 namespace mmx {
 
 MMX_POW2_LENGTHS(MMX_INST_XF)
-#define MMX_INST_XI_PLAIN(L) MMX_INST_XI(L, false)
+#define MMX_INST_XI_PLAIN(L) MMX_INST_XI(L, XI_UNIFORM)
 MMX_POW2_LENGTHS(MMX_INST_XI_PLAIN)
 #undef MMX_INST_XI_PLAIN
 

@@ -6,12 +6,12 @@
 
 namespace mmx {
 
-#define MMX_INST_XI_BOTH(L) MMX_INST_XI(L, false) MMX_INST_XI(L, true)
+#define MMX_INST_XI_ALL(L) MMX_INST_XI(L, XI_UNIFORM) MMX_INST_XI(L, XI_MAPS) MMX_INST_XI(L, XI_GENERAL)
 MMX_R5_SHORT(MMX_INST_XF)
-MMX_R5_SHORT(MMX_INST_XI_BOTH)
+MMX_R5_SHORT(MMX_INST_XI_ALL)
 MMX_R5_SHORT(MMX_INST_Z)
 MMX_R5_SHORT(MMX_INST_Y)
 MMX_R5_LONG(MMX_INST_Y)
-#undef MMX_INST_XI_BOTH
+#undef MMX_INST_XI_ALL
 
 } // namespace mmx

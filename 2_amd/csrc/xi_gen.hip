@@ -6,7 +6,7 @@
 
 namespace mmx {
 
-#define MMX_INST_XI_GEN(L) MMX_INST_XI(L, true)
+#define MMX_INST_XI_GEN(L) MMX_INST_XI(L, XI_GENERAL)
 MMX_POW2_LENGTHS(MMX_INST_XI_GEN)
 #undef MMX_INST_XI_GEN
 

@@ -7,6 +7,9 @@
 * the same with Msat, Aex, alpha and Ku maps on top; power-of-two x sizes
   run the register-FFT x pass with the general per-cell epilogue, the others
   the generic LDS passes: both against the oracle;
+* Msat / Aex maps as the only per-cell terms (with Msat and Aex holes, open
+  and periodic): the register x pass's two-cell epilogue with exchange
+  weights, against the oracle, H_eff, torque and RK12 tries;
 * the standalone uniaxialAnisotropyAsync with its u maps.
 """
 import numpy as np
@@ -172,3 +175,56 @@ def test_uniaxial_kernel_u_maps(mmx_gpu):
     want = O.uniaxial(m, ku2, np.float32(u).astype(np.float64), ku_map=kmap, msat_map=ms,
                       u_map=umap)
     assert relerr(got, want) < TOL
+
+
+# Msat / Aex maps as the only per-cell terms (uniform anisotropy, uniform
+# B_ext, no spin-transfer term): the register x pass runs the two-cell
+# epilogue with exchange weights; Msat holes (zero cells) check the masked
+# renormalisation and the anisotropy constant's zero guard.
+MAP_CASES = [((4, 16, 32), (0, 0, 0), False), ((2, 32, 64), (0, 0, 0), True),
+             ((2, 16, 64), (0, 0, 1), True), ((4, 16, 32), (0, 1, 0), True),
+             ((4, 8, 16), (1, 0, 0), False)]
+
+
+@pytest.mark.parametrize("n,pbc,aex_map", MAP_CASES)
+def test_msat_aex_maps_only(mmx_gpu, n, pbc, aex_map):
+    mmx = mmx_gpu
+    cell = (3e-9, 4e-9, 4e-9)
+    _say("msat/aex maps only", n, pbc, aex_map)
+    rng = np.random.default_rng(sum(n) + 7 * sum(pbc))
+    m = O.normalize(rng.normal(size=(3,) + n)).astype(np.float32)
+    ms = rng.uniform(0.6, 1.0, size=n).astype(np.float32)
+    ms[rng.uniform(size=n) < 0.1] = 0.0
+    bext = (0.004, -0.006, 0.009)
+    sim = mmx.Sim(n, cell, pbc)
+    sim.setv("Msat", 8e5)
+    sim.setv("Aex", 1.3e-11)
+    sim.setv("alpha", 0.05)
+    sim.setv("B_ext", bext)
+    sim.load("anisotropy/uniaxial")
+    sim.setv("Ku", 5e5)
+    sim.setv("anisU", (0.6, 0.0, 0.8))
+    sim.setmask("Msat", ms, 1)
+    kw = dict(msat_map=ms)
+    if aex_map:
+        ax = rng.uniform(0.5, 1.5, size=n).astype(np.float32)
+        ax[rng.uniform(size=n) < 0.05] = 0.0
+        sim.setmask("Aex", ax, 1)
+        kw.update(aex_map=ax)
+    sim.setarray("m", m)
+    prob = O.Problem(n, cell, pbc=pbc, msat=8e5, aex=1.3e-11, alpha=0.05, bext=bext, ku=5e5,
+                     anis_u=(0.6, 0.0, 0.8), **kw)
+    mo = prob.normalized(m.astype(np.float64).copy())
+    want_h = prob.heff(mo)
+    eh = relerr(sim.getarray("H_eff"), want_h)
+    et = relerr(sim.getarray("torque"), prob.torque(mo, want_h))
+    # the tries start from m as normalised with the Msat map (holes are 0)
+    for k in range(6):
+        dt = 1e-14 * (1 + 0.1 * k)
+        eg = sim.try_step(dt)
+        eo = prob.try_step(mo, dt)
+        assert abs(eg - eo) <= 1e-3 * eo + 1e-12, (k, eg, eo)
+    dm = np.abs(sim.getarray("m") - prob.normalized(mo)).max()
+    _say("maps only", n, pbc, "H_eff %.2e torque %.2e rk12 |dm| %.2e" % (eh, et, dm))
+    assert eh < TOL and et < TOL and dm < TOL
+    sim.close()
