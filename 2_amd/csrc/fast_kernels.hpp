@@ -840,8 +840,9 @@ enum XiVariant { XI_UNIFORM = 0, XI_MAPS = 1, XI_GENERAL = 2 };
 // Minimum waves per SIMD the xi kernel is built for.  The map and general
 // epilogues of the power-of-two lengths up to 256 take all 256 VGPRs (one
 // wave per SIMD) when left alone; bounded to two waves they spill a few
-// dozen registers and still run faster (256-point rows, Msat map: 146 ->
-// 103 us).  Longer rows, and the other lengths, spill hundreds at two waves.
+// dozen registers and still run faster (256-point rows at 256x256x64: Msat
+// map 146 -> 103 us, Zhang-Li 136 -> 127 us).  Longer rows, and the other
+// lengths, spill hundreds at two waves.
 template <int LX, int VAR> constexpr int xi_min_waves()
 {
     return (VAR != XI_UNIFORM && LX <= 256 && (LX & (LX - 1)) == 0) ? 2 : 1;

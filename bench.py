@@ -751,6 +751,71 @@ def run_demagexch(args):
     }
 
 
+def run_masked(args):
+    """The 256x256x64 bulk workload with per-cell terms, one configuration at
+    a time: uniform parameters (the headline), a cylinder geometry as an Msat
+    mask (256-cell disk through all 64 layers, zero outside), the same with an
+    Aex map on top, and uniform parameters with the Zhang-Li torque.  H_eff
+    evaluations/s, per-pass times and adaptive RK12 steps/s of each; the xi
+    pass is where the configurations differ (its three epilogue variants)."""
+    import numpy as np
+    import torch
+    import mmx
+    from mmx import synth
+
+    def say(*a):
+        print("[masked]", *a, file=sys.stderr, flush=True)
+
+    cfg = synth.config(2)
+    n = cfg["n"]
+    _, y, x = np.meshgrid(np.arange(n[0]), np.arange(n[1]), np.arange(n[2]), indexing="ij")
+    r2 = (x - (n[2] - 1) / 2.0) ** 2 + (y - (n[1] - 1) / 2.0) ** 2
+    disk = (r2 <= (n[2] / 2.0) ** 2).astype(np.float32)
+    aex = np.random.default_rng(5).uniform(0.5, 1.5, size=n).astype(np.float32)
+    rows = {}
+    for case in ("uniform", "msat_disk", "msat_disk_aex_map", "zhang_li"):
+        sim = mmx.Sim(n, cfg["cell"])
+        synth.setup_sim(sim, cfg)
+        if case.startswith("msat_disk"):
+            sim.setmask("Msat", disk, 1)
+        if case == "msat_disk_aex_map":
+            sim.setmask("Aex", aex, 1)
+        if case == "zhang_li":
+            sim.load("zhang-li")
+            sim.setv("j", (1e12, 0.0, 0.0))
+        sim.update("H_eff")
+        elapsed = _time_evals(sim, args.steps, args.warmup)
+        passes = sim.time_passes(reps=max(10, min(args.steps, 50)))
+        rk = None
+        if args.rk_steps > 0:
+            sim.setv("t", 0.0)
+            sim.setv("dt", 1e-15)
+            sim.step()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            sim.steps(args.rk_steps)
+            torch.cuda.synchronize()
+            rk = args.rk_steps / (time.perf_counter() - t0)
+        sim.close()
+        rows[case] = {"evals_per_s": args.steps / elapsed,
+                      "passes_us": {k: round(v * 1e3, 1) for k, v in passes.items()},
+                      "rk12_steps_per_s": rk}
+        say(case, "%.1f evals/s" % rows[case]["evals_per_s"], rows[case]["passes_us"],
+            "rk12 %s steps/s" % (None if rk is None else round(rk, 1)))
+    head = rows["msat_disk"]
+    return {
+        "metric": "H_eff evals/s at 256^2x64 FP32 with an Msat-masked cylinder geometry",
+        "value": head["evals_per_s"], "unit": "H_eff evals/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 / head["evals_per_s"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "bulk 256x256x64 cells, 4 nm, Ku along z, seeded random m; "
+                               "Msat mask: a 256-cell disk through all 64 layers (zero outside)",
+                   "steps_are": "H_eff evaluations"},
+        "configurations": rows,
+    }
+
+
 def main():
     if len(sys.argv) > 1 and sys.argv[1] == "--summarize":
         print(summarize_profiles(*sys.argv[2:5]))
@@ -762,13 +827,14 @@ def main():
     ap.add_argument("--rk-steps", type=int, default=1000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", choices=("bulk", "film", "precision", "batch", "surrogate",
-                                           "surrogate-kernels", "demagexch"),
+                                           "surrogate-kernels", "demagexch", "masked"),
                     default="bulk",
                     help="bulk: 256x256x64 (the headline line); film: 128x32x1 RK12 for 1 ns "
                          "beside the oracle's RK12; precision: 1024x1024x32 film with the FP64 "
                          "Newell kernel and per-field errors against the FP64 oracle; batch: 64 "
                          "uneven films; surrogate: training and surrogate-assisted runs; "
-                         "surrogate-kernels: the surrogate's kernels alone; demagexch")
+                         "surrogate-kernels: the surrogate's kernels alone; demagexch; masked: "
+                         "the bulk grid with an Msat-masked cylinder, an Aex map, Zhang-Li")
     ap.add_argument("--train-samples", type=int, default=20000,
                     help="surrogate workload: training pairs from exact trajectories")
     ap.add_argument("--sim-time", type=float, default=1e-9,
@@ -808,6 +874,9 @@ def main():
         return
     if args.workload == "demagexch":
         print(json.dumps(run_demagexch(args)))
+        return
+    if args.workload == "masked":
+        print(json.dumps(run_masked(args)))
         return
     cfg = synth.config(2)
     sim = mmx.Sim(cfg["n"], cfg["cell"])
