@@ -786,22 +786,25 @@ def run_masked(args):
         sim.update("H_eff")
         elapsed = _time_evals(sim, args.steps, args.warmup)
         passes = sim.time_passes(reps=max(10, min(args.steps, 50)))
-        rk = None
+        rk = per_step = None
         if args.rk_steps > 0:
             sim.setv("t", 0.0)
             sim.setv("dt", 1e-15)
             sim.step()
             torch.cuda.synchronize()
+            e0 = sim.getv("evals")
             t0 = time.perf_counter()
             sim.steps(args.rk_steps)
             torch.cuda.synchronize()
             rk = args.rk_steps / (time.perf_counter() - t0)
+            per_step = (sim.getv("evals") - e0) / args.rk_steps
         sim.close()
         rows[case] = {"evals_per_s": args.steps / elapsed,
                       "passes_us": {k: round(v * 1e3, 1) for k, v in passes.items()},
-                      "rk12_steps_per_s": rk}
+                      "rk12_steps_per_s": rk, "rk12_evals_per_step": per_step}
         say(case, "%.1f evals/s" % rows[case]["evals_per_s"], rows[case]["passes_us"],
-            "rk12 %s steps/s" % (None if rk is None else round(rk, 1)))
+            "rk12 %s steps/s, %s evals/step" % (None if rk is None else round(rk, 1),
+                                                None if per_step is None else round(per_step, 2)))
     head = rows["msat_disk"]
     return {
         "metric": "H_eff evals/s at 256^2x64 FP32 with an Msat-masked cylinder geometry",
