@@ -663,10 +663,10 @@ xf_fast(DemagGeom g, const float2 *__restrict__ tw, const float2 *__restrict__ r
 }
 
 // Two neighbouring cells (k0 even, k0 + 1 < n2) of one row at once, for
-// uniform material parameters (MAPS false) or Msat / Aex maps only (MAPS
-// true), and no spin-transfer term: the same per-cell arithmetic as
-// cell_epilogue (exchange6 order dim 0, dim 2, dim 1; SumUpdater order), with
-// 8-byte loads and stores of the cell pair.
+// uniform material parameters (MAPS false) or the scalar maps Msat, Aex,
+// alpha and Ku (MAPS true), and no spin-transfer term: the same per-cell
+// arithmetic as cell_epilogue (exchange6 order dim 0, dim 2, dim 1;
+// SumUpdater order), with 8-byte loads and stores of the cell pair.
 
 __device__ __forceinline__ int nb_lo(int pos, int len, int wrap)
 {
@@ -784,6 +784,11 @@ __device__ __forceinline__ void pair_epilogue(const DemagGeom &g, const FieldPar
         if constexpr (MAPS) {
             k = fp.ku2 / (ms.x == 0.0f ? 1.0f : ms.x);
             k1 = fp.ku2 / (ms.y == 0.0f ? 1.0f : ms.y);
+            if (fp.ku_map) {
+                const float2 q = ld_off<float2>(fp.ku_map, b0);
+                k *= q.x;
+                k1 *= q.y;
+            }
         }
         const float mu0 = m0[0] * fp.u[0] + m0[1] * fp.u[1] + m0[2] * fp.u[2];
         const float mu1 = m1[0] * fp.u[0] + m1[1] * fp.u[1] + m1[2] * fp.u[2];
@@ -799,8 +804,15 @@ __device__ __forceinline__ void pair_epilogue(const DemagGeom &g, const FieldPar
     }
     if (MODE == EPI_HEFF && io.tout == nullptr) return;
     float t0[3], t1[3];
-    llg_torque(m0, H0, fp.alpha, t0);
-    llg_torque(m1, H1, fp.alpha, t1);
+    float2 alpha = float2{fp.alpha, fp.alpha};
+    if constexpr (MAPS) {
+        if (fp.alpha_map) {
+            const float2 a = ld_off<float2>(fp.alpha_map, b0);
+            alpha = float2{fp.alpha * a.x, fp.alpha * a.y};
+        }
+    }
+    llg_torque(m0, H0, alpha.x, t0);
+    llg_torque(m1, H1, alpha.y, t1);
     if (io.tout) {
 #pragma unroll
         for (int c = 0; c < 3; c++) st_off<float2>(io.tout + c * N, b0, float2{t0[c], t1[c]});
@@ -831,8 +843,9 @@ __device__ __forceinline__ void pair_epilogue(const DemagGeom &g, const FieldPar
 // xi: C2R along x (imaginary parts of DC and Nyquist ignored), keep k < n2,
 // then the fused field / torque / RK epilogue, one of three variants:
 // XI_UNIFORM the pair epilogue for uniform parameters, XI_MAPS the pair
-// epilogue with Msat / Aex maps, XI_GENERAL the per-cell epilogue
-// (cell_epilogue: every map and mask, Zhang-Li, the thermal field).  Each is
+// epilogue with the scalar maps (Msat, Aex, alpha, Ku), XI_GENERAL the
+// per-cell epilogue (cell_epilogue: every map and mask, Zhang-Li, the
+// thermal field).  Each is
 // its own kernel: the general epilogue's register demand would otherwise set
 // the occupancy of the pair variants.
 enum XiVariant { XI_UNIFORM = 0, XI_MAPS = 1, XI_GENERAL = 2 };

@@ -1,5 +1,5 @@
 // xi_maps.hip -- the xi pass of the power-of-two lengths with the two-cell
-// epilogue for Msat / Aex maps (the only per-cell terms), in a file of its
+// epilogue for the scalar maps (Msat, Aex, alpha, Ku), in a file of its
 // own so that it compiles in parallel with the other passes.
 #include "fast_launch.hpp"
 

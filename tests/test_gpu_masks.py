@@ -7,9 +7,10 @@
 * the same with Msat, Aex, alpha and Ku maps on top; power-of-two x sizes
   run the register-FFT x pass with the general per-cell epilogue, the others
   the generic LDS passes: both against the oracle;
-* Msat / Aex maps as the only per-cell terms (with Msat and Aex holes, open
-  and periodic): the register x pass's two-cell epilogue with exchange
-  weights, against the oracle, H_eff, torque and RK12 tries;
+* the scalar maps (Msat, Aex, alpha, Ku) as the only per-cell terms (with
+  Msat and Aex holes, open and periodic): the register x pass's two-cell
+  epilogue with exchange weights, against the oracle, H_eff, torque and RK12
+  tries;
 * the standalone uniaxialAnisotropyAsync with its u maps.
 """
 import numpy as np
@@ -177,24 +178,24 @@ def test_uniaxial_kernel_u_maps(mmx_gpu):
     assert relerr(got, want) < TOL
 
 
-# Msat / Aex maps as the only per-cell terms (uniform anisotropy, uniform
-# B_ext, no spin-transfer term): the register x pass runs the two-cell
-# epilogue with exchange weights; Msat holes (zero cells) check the masked
-# renormalisation and the anisotropy constant's zero guard.
-MAP_CASES = [((4, 16, 32), (0, 0, 0), False), ((2, 32, 64), (0, 0, 0), True),
-             ((2, 16, 64), (0, 0, 1), True), ((4, 16, 32), (0, 1, 0), True),
-             ((4, 8, 16), (1, 0, 0), False)]
+# Scalar maps (Msat, Aex, alpha, Ku) as the only per-cell terms (uniform
+# anisU, uniform B_ext, no spin-transfer term): the register x pass runs the
+# two-cell epilogue with exchange weights; Msat holes (zero cells) check the
+# masked renormalisation and the anisotropy constant's zero guard.
+# maps: "msat", "msat+aex", "all" (Msat, Aex, alpha, Ku), "alpha+ku" (no Msat)
+MAP_CASES = [((4, 16, 32), (0, 0, 0), "msat"), ((2, 32, 64), (0, 0, 0), "msat+aex"),
+             ((2, 16, 64), (0, 0, 1), "all"), ((4, 16, 32), (0, 1, 0), "msat+aex"),
+             ((4, 8, 16), (1, 0, 0), "msat"), ((2, 32, 32), (0, 0, 0), "all"),
+             ((4, 16, 32), (0, 0, 0), "alpha+ku")]
 
 
-@pytest.mark.parametrize("n,pbc,aex_map", MAP_CASES)
-def test_msat_aex_maps_only(mmx_gpu, n, pbc, aex_map):
+@pytest.mark.parametrize("n,pbc,maps", MAP_CASES)
+def test_scalar_maps_only(mmx_gpu, n, pbc, maps):
     mmx = mmx_gpu
     cell = (3e-9, 4e-9, 4e-9)
-    _say("msat/aex maps only", n, pbc, aex_map)
-    rng = np.random.default_rng(sum(n) + 7 * sum(pbc))
+    _say("scalar maps only", n, pbc, maps)
+    rng = np.random.default_rng(sum(n) + 7 * sum(pbc) + len(maps))
     m = O.normalize(rng.normal(size=(3,) + n)).astype(np.float32)
-    ms = rng.uniform(0.6, 1.0, size=n).astype(np.float32)
-    ms[rng.uniform(size=n) < 0.1] = 0.0
     bext = (0.004, -0.006, 0.009)
     sim = mmx.Sim(n, cell, pbc)
     sim.setv("Msat", 8e5)
@@ -204,13 +205,22 @@ def test_msat_aex_maps_only(mmx_gpu, n, pbc, aex_map):
     sim.load("anisotropy/uniaxial")
     sim.setv("Ku", 5e5)
     sim.setv("anisU", (0.6, 0.0, 0.8))
-    sim.setmask("Msat", ms, 1)
-    kw = dict(msat_map=ms)
-    if aex_map:
+    kw = {}
+    if maps != "alpha+ku":
+        ms = rng.uniform(0.6, 1.0, size=n).astype(np.float32)
+        ms[rng.uniform(size=n) < 0.1] = 0.0
+        sim.setmask("Msat", ms, 1)
+        kw.update(msat_map=ms)
+    if maps in ("msat+aex", "all"):
         ax = rng.uniform(0.5, 1.5, size=n).astype(np.float32)
         ax[rng.uniform(size=n) < 0.05] = 0.0
         sim.setmask("Aex", ax, 1)
         kw.update(aex_map=ax)
+    if maps in ("all", "alpha+ku"):
+        for name, key in (("alpha", "alpha_map"), ("Ku", "ku_map")):
+            a = rng.uniform(0.5, 1.5, size=n).astype(np.float32)
+            sim.setmask(name, a, 1)
+            kw[key] = a
     sim.setarray("m", m)
     prob = O.Problem(n, cell, pbc=pbc, msat=8e5, aex=1.3e-11, alpha=0.05, bext=bext, ku=5e5,
                      anis_u=(0.6, 0.0, 0.8), **kw)
@@ -225,6 +235,6 @@ def test_msat_aex_maps_only(mmx_gpu, n, pbc, aex_map):
         eo = prob.try_step(mo, dt)
         assert abs(eg - eo) <= 1e-3 * eo + 1e-12, (k, eg, eo)
     dm = np.abs(sim.getarray("m") - prob.normalized(mo)).max()
-    _say("maps only", n, pbc, "H_eff %.2e torque %.2e rk12 |dm| %.2e" % (eh, et, dm))
+    _say("maps only", n, pbc, maps, "H_eff %.2e torque %.2e rk12 |dm| %.2e" % (eh, et, dm))
     assert eh < TOL and et < TOL and dm < TOL
     sim.close()
