@@ -664,9 +664,10 @@ xf_fast(DemagGeom g, const float2 *__restrict__ tw, const float2 *__restrict__ r
 
 // Two neighbouring cells (k0 even, k0 + 1 < n2) of one row at once, for
 // uniform material parameters (MAPS false) or the scalar maps Msat, Aex,
-// alpha and Ku (MAPS true), and no spin-transfer term: the same per-cell
-// arithmetic as cell_epilogue (exchange6 order dim 0, dim 2, dim 1;
-// SumUpdater order), with 8-byte loads and stores of the cell pair.
+// alpha and Ku and the thermal field (MAPS true), and no spin-transfer term:
+// the same per-cell arithmetic as cell_epilogue (exchange6 order dim 0,
+// dim 2, dim 1; SumUpdater order), with 8-byte loads and stores of the cell
+// pair.
 
 __device__ __forceinline__ int nb_lo(int pos, int len, int wrap)
 {
@@ -798,6 +799,17 @@ __device__ __forceinline__ void pair_epilogue(const DemagGeom &g, const FieldPar
             H1[c] += k1 * mu1 * fp.u[c];
         }
     }
+    // H_therm, the SumUpdater's last term
+    if constexpr (MAPS) {
+        if (fp.htherm) {
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                const float2 t = ld_off<float2>(fp.htherm + c * N, b0);
+                H0[c] += t.x;
+                H1[c] += t.y;
+            }
+        }
+    }
     if (io.hout) {
 #pragma unroll
         for (int c = 0; c < 3; c++) st_off<float2>(io.hout + c * N, b0, float2{H0[c], H1[c]});
@@ -843,9 +855,9 @@ __device__ __forceinline__ void pair_epilogue(const DemagGeom &g, const FieldPar
 // xi: C2R along x (imaginary parts of DC and Nyquist ignored), keep k < n2,
 // then the fused field / torque / RK epilogue, one of three variants:
 // XI_UNIFORM the pair epilogue for uniform parameters, XI_MAPS the pair
-// epilogue with the scalar maps (Msat, Aex, alpha, Ku), XI_GENERAL the
-// per-cell epilogue (cell_epilogue: every map and mask, Zhang-Li, the
-// thermal field).  Each is
+// epilogue with the scalar maps (Msat, Aex, alpha, Ku) and the thermal
+// field, XI_GENERAL the per-cell epilogue (cell_epilogue: every map and
+// mask, Zhang-Li, the thermal field, demagexch).  Each is
 // its own kernel: the general epilogue's register demand would otherwise set
 // the occupancy of the pair variants.
 enum XiVariant { XI_UNIFORM = 0, XI_MAPS = 1, XI_GENERAL = 2 };

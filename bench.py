@@ -755,7 +755,8 @@ def run_masked(args):
     """The 256x256x64 bulk workload with per-cell terms, one configuration at
     a time: uniform parameters (the headline), a cylinder geometry as an Msat
     mask (256-cell disk through all 64 layers, zero outside), the same with an
-    Aex map on top, and uniform parameters with the Zhang-Li torque.  H_eff
+    Aex map on top, uniform parameters with the Zhang-Li torque, and with the
+    Brown thermal field at 300 K (redrawn every RK12 step).  H_eff
     evaluations/s, per-pass times and adaptive RK12 steps/s of each; the xi
     pass is where the configurations differ (its three epilogue variants)."""
     import numpy as np
@@ -773,7 +774,7 @@ def run_masked(args):
     disk = (r2 <= (n[2] / 2.0) ** 2).astype(np.float32)
     aex = np.random.default_rng(5).uniform(0.5, 1.5, size=n).astype(np.float32)
     rows = {}
-    for case in ("uniform", "msat_disk", "msat_disk_aex_map", "zhang_li"):
+    for case in ("uniform", "msat_disk", "msat_disk_aex_map", "zhang_li", "thermal_300K"):
         sim = mmx.Sim(n, cfg["cell"])
         synth.setup_sim(sim, cfg)
         if case.startswith("msat_disk"):
@@ -783,6 +784,9 @@ def run_masked(args):
         if case == "zhang_li":
             sim.load("zhang-li")
             sim.setv("j", (1e12, 0.0, 0.0))
+        if case == "thermal_300K":
+            sim.load("temperature/brown")
+            sim.setv("Temp", 300.0)
         sim.update("H_eff")
         elapsed = _time_evals(sim, args.steps, args.warmup)
         passes = sim.time_passes(reps=max(10, min(args.steps, 50)))
@@ -837,7 +841,8 @@ def main():
                          "Newell kernel and per-field errors against the FP64 oracle; batch: 64 "
                          "uneven films; surrogate: training and surrogate-assisted runs; "
                          "surrogate-kernels: the surrogate's kernels alone; demagexch; masked: "
-                         "the bulk grid with an Msat-masked cylinder, an Aex map, Zhang-Li")
+                         "the bulk grid with an Msat-masked cylinder, an Aex map, Zhang-Li, "
+                         "the thermal field")
     ap.add_argument("--train-samples", type=int, default=20000,
                     help="surrogate workload: training pairs from exact trajectories")
     ap.add_argument("--sim-time", type=float, default=1e-9,

@@ -7,7 +7,9 @@
   Box-Muller step;
 * solver/heun with the thermal field replayed by the oracle (draw k in step
   k, the field held over the step): m after every step to 1e-5;
-* a new Therm_seed restarts the draws; Temp = 0 removes the field.
+* a new Therm_seed restarts the draws; Temp = 0 removes the field;
+* the same Heun replay on the register-FFT x pass (its two-cell epilogue
+  adds the field), and with maps against the generic passes.
 """
 import ctypes as C
 
@@ -130,6 +132,60 @@ def test_heun_with_thermal_field_against_oracle(mmx_gpu):
     want = T.htherm(N3, CELL, 3, 6, 300.0, 0.1, 8e5, O.GAMMA0, dt)
     assert np.abs(s.getarray("H_therm") - want).max() <= 1e-6 * np.abs(want).max()
     s.close()
+
+
+@pytest.fixture
+def register_x_passes(mmx_gpu):
+    """The register-FFT x passes at this grid size (by default grids of fewer
+    than 256 x-pass workgroups take the generic passes)."""
+    lib = mmx_gpu.lib()
+    prev = C.c_int()
+    mmx_gpu.check(lib.mmx_set_fast_x_min_blocks(0, C.byref(prev)))
+    yield
+    mmx_gpu.check(lib.mmx_set_fast_x_min_blocks(prev.value, None))
+
+
+def test_heun_thermal_on_register_x_pass(mmx_gpu, register_x_passes):
+    """The thermal field in the register x pass's two-cell epilogue: heun
+    against the oracle as above, and with Temp/alpha/Msat maps against the
+    generic passes' per-cell epilogue on the same GPU."""
+    _say("heun + H_therm, register x pass", N3)
+    dt = 1e-14
+    s = _sim(mmx_gpu, maps=False, seed=3)
+    s.load("solver/heun")
+    s.setv("dt", dt)
+    s.setv("B_ext", (0.0, 0.004, 0.009))
+    rng = np.random.default_rng(6)
+    m = O.normalize(rng.normal(size=(3,) + N3))
+    s.setarray("m", m.astype(np.float32))
+    prob = O.Problem(N3, CELL, msat=8e5, aex=1.3e-11, alpha=0.1, bext=(0.0, 0.004, 0.009))
+    mo = m.astype(np.float32).astype(np.float64)
+    for k in range(6):
+        s.step()
+        ht = T.htherm(N3, CELL, 3, k, 300.0, 0.1, 8e5, O.GAMMA0, dt).astype(np.float64)
+        mo = T.heun_step(prob, mo, dt, ht, O.GAMMA0)
+        err = np.abs(s.getarray("m") - mo).max()
+        _say("step %d max |dm| %.2e" % (k, err))
+        assert err < 1e-5
+    s.close()
+    # with maps: register x pass against the generic passes
+    lib = mmx_gpu.lib()
+    out = []
+    for threshold in (0, 1 << 30):
+        mmx_gpu.check(lib.mmx_set_fast_x_min_blocks(threshold, None))
+        s = _sim(mmx_gpu, maps=True, seed=4)
+        s.load("solver/heun")
+        s.setv("dt", dt)
+        s.setv("B_ext", (0.0, 0.004, 0.009))
+        s.setarray("m", m.astype(np.float32))
+        s.steps(6)
+        out.append((s.getarray("m"), s.getarray("H_eff")))
+        s.close()
+    mmx_gpu.check(lib.mmx_set_fast_x_min_blocks(0, None))
+    dm = np.abs(out[0][0] - out[1][0]).max()
+    dh = np.abs(out[0][1] - out[1][1]).max() / np.abs(out[1][1]).max()
+    _say("maps: register vs generic |dm| %.2e, H_eff %.2e" % (dm, dh))
+    assert dm < 1e-5 and dh < 1e-5
 
 
 def test_temperature_changes_the_trajectory_only_through_the_field(mmx_gpu):
