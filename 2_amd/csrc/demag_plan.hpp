@@ -161,10 +161,10 @@ bool fast_xf_used(const DemagPlanImpl *P);
 int fast_xi(const DemagPlanImpl *P, int mode, const FieldParams &fp, const EpilogueIO &io,
             hipStream_t st);
 // The fast C2R pass's two-cell epilogue covers uniform parameters
-// (fast_xi_applies) and the scalar maps Msat, Aex, alpha and Ku with the
-// thermal field (fast_xi_maps_applies: no B_ext or anisU mask, no
-// spin-transfer term, no demag-exchange field); otherwise fast_xi runs the
-// general per-cell epilogue (xi_gen.hip).
+// (fast_xi_applies) and the Msat, Aex, alpha and Ku maps, the B_ext and
+// anisU masks and the thermal field (fast_xi_maps_applies: no spin-transfer
+// term, no demag-exchange field); otherwise fast_xi runs the general
+// per-cell epilogue (xi_gen.hip).
 bool fast_xi_applies(const FieldParams &fp);
 bool fast_xi_maps_applies(const FieldParams &fp);
 

@@ -663,11 +663,11 @@ xf_fast(DemagGeom g, const float2 *__restrict__ tw, const float2 *__restrict__ r
 }
 
 // Two neighbouring cells (k0 even, k0 + 1 < n2) of one row at once, for
-// uniform material parameters (MAPS false) or the scalar maps Msat, Aex,
-// alpha and Ku and the thermal field (MAPS true), and no spin-transfer term:
-// the same per-cell arithmetic as cell_epilogue (exchange6 order dim 0,
-// dim 2, dim 1; SumUpdater order), with 8-byte loads and stores of the cell
-// pair.
+// uniform material parameters (MAPS false) or the Msat, Aex, alpha and Ku
+// maps, the B_ext and anisU masks and the thermal field (MAPS true), and no
+// spin-transfer term: the same per-cell arithmetic as cell_epilogue
+// (exchange6 order dim 0, dim 2, dim 1; SumUpdater order), with 8-byte loads
+// and stores of the cell pair.
 
 __device__ __forceinline__ int nb_lo(int pos, int len, int wrap)
 {
@@ -754,8 +754,15 @@ __device__ __forceinline__ void pair_epilogue(const DemagGeom &g, const FieldPar
             e1 += p2 * (w2l1 * (m0[c] - m1[c]) + w2h1 * (xh - m1[c]));
             e0 += p1 * (w1l0 * (yl.x - m0[c]) + w1h0 * (yh.x - m0[c]));
             e1 += p1 * (w1l1 * (yl.y - m1[c]) + w1h1 * (yh.y - m1[c]));
-            H0[c] = hd0[c] + e0 + fp.hext[c];
-            H1[c] = hd1[c] + e1 + fp.hext[c];
+            // B_ext mask: multiplier * map (the SumUpdater's vecMadd)
+            float hx0 = fp.hext[c], hx1 = fp.hext[c];
+            if (fp.bext_map) {
+                const float2 bm = ld_off<float2>(fp.bext_map + c * N, b0);
+                hx0 = fp.hext[c] * bm.x;
+                hx1 = fp.hext[c] * bm.y;
+            }
+            H0[c] = hd0[c] + e0 + hx0;
+            H1[c] = hd1[c] + e1 + hx1;
         }
     } else {
 #pragma unroll
@@ -791,12 +798,24 @@ __device__ __forceinline__ void pair_epilogue(const DemagGeom &g, const FieldPar
                 k1 *= q.y;
             }
         }
-        const float mu0 = m0[0] * fp.u[0] + m0[1] * fp.u[1] + m0[2] * fp.u[2];
-        const float mu1 = m1[0] * fp.u[0] + m1[1] * fp.u[1] + m1[2] * fp.u[2];
+        // the anisU mask gives each cell's axis as multiplier * map
+        float u0[3] = {fp.u[0], fp.u[1], fp.u[2]}, u1[3] = {fp.u[0], fp.u[1], fp.u[2]};
+        if constexpr (MAPS) {
+            if (fp.u_map) {
+#pragma unroll
+                for (int c = 0; c < 3; c++) {
+                    const float2 um = ld_off<float2>(fp.u_map + c * N, b0);
+                    u0[c] = fp.u[c] * um.x;
+                    u1[c] = fp.u[c] * um.y;
+                }
+            }
+        }
+        const float mu0 = m0[0] * u0[0] + m0[1] * u0[1] + m0[2] * u0[2];
+        const float mu1 = m1[0] * u1[0] + m1[1] * u1[1] + m1[2] * u1[2];
 #pragma unroll
         for (int c = 0; c < 3; c++) {
-            H0[c] += k * mu0 * fp.u[c];
-            H1[c] += k1 * mu1 * fp.u[c];
+            H0[c] += k * mu0 * u0[c];
+            H1[c] += k1 * mu1 * u1[c];
         }
     }
     // H_therm, the SumUpdater's last term
@@ -855,9 +874,9 @@ __device__ __forceinline__ void pair_epilogue(const DemagGeom &g, const FieldPar
 // xi: C2R along x (imaginary parts of DC and Nyquist ignored), keep k < n2,
 // then the fused field / torque / RK epilogue, one of three variants:
 // XI_UNIFORM the pair epilogue for uniform parameters, XI_MAPS the pair
-// epilogue with the scalar maps (Msat, Aex, alpha, Ku) and the thermal
-// field, XI_GENERAL the per-cell epilogue (cell_epilogue: every map and
-// mask, Zhang-Li, the thermal field, demagexch).  Each is
+// epilogue with maps and masks (Msat, Aex, alpha, Ku, B_ext, anisU) and the
+// thermal field, XI_GENERAL the per-cell epilogue (cell_epilogue: all of
+// these plus Zhang-Li and demagexch).  Each is
 // its own kernel: the general epilogue's register demand would otherwise set
 // the occupancy of the pair variants.
 enum XiVariant { XI_UNIFORM = 0, XI_MAPS = 1, XI_GENERAL = 2 };

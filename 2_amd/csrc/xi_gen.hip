@@ -1,8 +1,7 @@
 // xi_gen.hip -- the xi pass of the power-of-two lengths with the general
-// per-cell epilogue (the B_ext and anisU masks, Zhang-Li with its
-// polarisation and xi masks, demagexch, with or without the scalar maps and
-// the thermal field), in a file of its own so that it compiles in parallel
-// with demag_fast.hip.
+// per-cell epilogue (Zhang-Li with its polarisation and xi masks, demagexch,
+// with or without the other maps, masks and the thermal field), in a file of
+// its own so that it compiles in parallel with demag_fast.hip.
 #include "fast_launch.hpp"
 
 namespace mmx {

@@ -755,7 +755,8 @@ def run_masked(args):
     """The 256x256x64 bulk workload with per-cell terms, one configuration at
     a time: uniform parameters (the headline), a cylinder geometry as an Msat
     mask (256-cell disk through all 64 layers, zero outside), the same with an
-    Aex map on top, uniform parameters with the Zhang-Li torque, and with the
+    Aex map on top, the same with B_ext masked to an 8-cell stripe across x
+    (an antenna), uniform parameters with the Zhang-Li torque, and with the
     Brown thermal field at 300 K (redrawn every RK12 step).  H_eff
     evaluations/s, per-pass times and adaptive RK12 steps/s of each; the xi
     pass is where the configurations differ (its three epilogue variants)."""
@@ -774,13 +775,18 @@ def run_masked(args):
     disk = (r2 <= (n[2] / 2.0) ** 2).astype(np.float32)
     aex = np.random.default_rng(5).uniform(0.5, 1.5, size=n).astype(np.float32)
     rows = {}
-    for case in ("uniform", "msat_disk", "msat_disk_aex_map", "zhang_li", "thermal_300K"):
+    stripe = np.zeros((3,) + n, np.float32)
+    stripe[:, :, :, n[2] // 2 - 4:n[2] // 2 + 4] = 1.0
+    for case in ("uniform", "msat_disk", "msat_disk_aex_map", "msat_disk_bext_stripe",
+                 "zhang_li", "thermal_300K"):
         sim = mmx.Sim(n, cfg["cell"])
         synth.setup_sim(sim, cfg)
         if case.startswith("msat_disk"):
             sim.setmask("Msat", disk, 1)
         if case == "msat_disk_aex_map":
             sim.setmask("Aex", aex, 1)
+        if case == "msat_disk_bext_stripe":
+            sim.setmask("B_ext", stripe, 3)
         if case == "zhang_li":
             sim.load("zhang-li")
             sim.setv("j", (1e12, 0.0, 0.0))

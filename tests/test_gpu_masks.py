@@ -7,10 +7,10 @@
 * the same with Msat, Aex, alpha and Ku maps on top; power-of-two x sizes
   run the register-FFT x pass with the general per-cell epilogue, the others
   the generic LDS passes: both against the oracle;
-* the scalar maps (Msat, Aex, alpha, Ku) as the only per-cell terms (with
-  Msat and Aex holes, open and periodic): the register x pass's two-cell
-  epilogue with exchange weights, against the oracle, H_eff, torque and RK12
-  tries;
+* the maps (Msat, Aex, alpha, Ku) and the B_ext / anisU masks without the
+  spin-transfer term (with Msat and Aex holes, open and periodic): the
+  register x pass's two-cell epilogue with exchange weights, against the
+  oracle, H_eff, torque and RK12 tries;
 * the standalone uniaxialAnisotropyAsync with its u maps.
 """
 import numpy as np
@@ -178,19 +178,21 @@ def test_uniaxial_kernel_u_maps(mmx_gpu):
     assert relerr(got, want) < TOL
 
 
-# Scalar maps (Msat, Aex, alpha, Ku) as the only per-cell terms (uniform
-# anisU, uniform B_ext, no spin-transfer term): the register x pass runs the
-# two-cell epilogue with exchange weights; Msat holes (zero cells) check the
-# masked renormalisation and the anisotropy constant's zero guard.
-# maps: "msat", "msat+aex", "all" (Msat, Aex, alpha, Ku), "alpha+ku" (no Msat)
+# Maps (Msat, Aex, alpha, Ku) and the B_ext / anisU masks without the
+# spin-transfer term: the register x pass runs the two-cell epilogue with
+# exchange weights; Msat holes (zero cells) check the masked renormalisation
+# and the anisotropy constant's zero guard.
+# maps: "msat", "msat+aex", "all" (Msat, Aex, alpha, Ku), "alpha+ku" (no Msat),
+# "masks" (the B_ext and anisU masks alone), "all+masks"
 MAP_CASES = [((4, 16, 32), (0, 0, 0), "msat"), ((2, 32, 64), (0, 0, 0), "msat+aex"),
              ((2, 16, 64), (0, 0, 1), "all"), ((4, 16, 32), (0, 1, 0), "msat+aex"),
              ((4, 8, 16), (1, 0, 0), "msat"), ((2, 32, 32), (0, 0, 0), "all"),
-             ((4, 16, 32), (0, 0, 0), "alpha+ku")]
+             ((4, 16, 32), (0, 0, 0), "alpha+ku"), ((4, 16, 32), (0, 0, 0), "masks"),
+             ((2, 16, 64), (0, 0, 1), "all+masks"), ((4, 16, 32), (0, 1, 0), "all+masks")]
 
 
 @pytest.mark.parametrize("n,pbc,maps", MAP_CASES)
-def test_scalar_maps_only(mmx_gpu, n, pbc, maps):
+def test_maps_without_spin_transfer(mmx_gpu, n, pbc, maps):
     mmx = mmx_gpu
     cell = (3e-9, 4e-9, 4e-9)
     _say("scalar maps only", n, pbc, maps)
@@ -205,24 +207,30 @@ def test_scalar_maps_only(mmx_gpu, n, pbc, maps):
     sim.load("anisotropy/uniaxial")
     sim.setv("Ku", 5e5)
     sim.setv("anisU", (0.6, 0.0, 0.8))
-    kw = {}
-    if maps != "alpha+ku":
+    kw = dict(bext=bext)
+    if maps in ("masks", "all+masks"):
+        bmap = rng.uniform(-1.0, 2.0, size=(3,) + n).astype(np.float32)
+        umap = rng.normal(size=(3,) + n).astype(np.float32)
+        sim.setmask("B_ext", bmap, 3)
+        sim.setmask("anisU", umap, 3)
+        kw.update(bext_map=bmap, anis_u_map=umap)
+    if maps not in ("alpha+ku", "masks"):
         ms = rng.uniform(0.6, 1.0, size=n).astype(np.float32)
         ms[rng.uniform(size=n) < 0.1] = 0.0
         sim.setmask("Msat", ms, 1)
         kw.update(msat_map=ms)
-    if maps in ("msat+aex", "all"):
+    if maps in ("msat+aex", "all", "all+masks"):
         ax = rng.uniform(0.5, 1.5, size=n).astype(np.float32)
         ax[rng.uniform(size=n) < 0.05] = 0.0
         sim.setmask("Aex", ax, 1)
         kw.update(aex_map=ax)
-    if maps in ("all", "alpha+ku"):
+    if maps in ("all", "alpha+ku", "all+masks"):
         for name, key in (("alpha", "alpha_map"), ("Ku", "ku_map")):
             a = rng.uniform(0.5, 1.5, size=n).astype(np.float32)
             sim.setmask(name, a, 1)
             kw[key] = a
     sim.setarray("m", m)
-    prob = O.Problem(n, cell, pbc=pbc, msat=8e5, aex=1.3e-11, alpha=0.05, bext=bext, ku=5e5,
+    prob = O.Problem(n, cell, pbc=pbc, msat=8e5, aex=1.3e-11, alpha=0.05, ku=5e5,
                      anis_u=(0.6, 0.0, 0.8), **kw)
     mo = prob.normalized(m.astype(np.float64).copy())
     want_h = prob.heff(mo)
