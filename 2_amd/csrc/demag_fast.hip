@@ -146,10 +146,10 @@ int fast_x_blocks(const DemagGeom &g)
 bool fast_xi_applies(const FieldParams &fp)
 {
     return !fp.msat_map && !fp.aex_map && !fp.alpha_map && !fp.ku_map && !fp.htherm &&
-           !fp.bext_map && !fp.u_map && fast_xi_maps_applies(fp);
+           !fp.bext_map && !fp.u_map && !fp.hdex && fast_xi_maps_applies(fp);
 }
 
-bool fast_xi_maps_applies(const FieldParams &fp) { return !fp.zl && !fp.hdex; }
+bool fast_xi_maps_applies(const FieldParams &fp) { return !fp.zl; }
 
 int fast_xi(const DemagPlanImpl *P, int mode, const FieldParams &fp, const EpilogueIO &io,
             hipStream_t st)

@@ -162,8 +162,8 @@ int fast_xi(const DemagPlanImpl *P, int mode, const FieldParams &fp, const Epilo
             hipStream_t st);
 // The fast C2R pass's two-cell epilogue covers uniform parameters
 // (fast_xi_applies) and the Msat, Aex, alpha and Ku maps, the B_ext and
-// anisU masks and the thermal field (fast_xi_maps_applies: no spin-transfer
-// term, no demag-exchange field); otherwise fast_xi runs the general
+// anisU masks, the thermal field and H_dex (fast_xi_maps_applies: no
+// spin-transfer term); with the Zhang-Li torque fast_xi runs the general
 // per-cell epilogue (xi_gen.hip).
 bool fast_xi_applies(const FieldParams &fp);
 bool fast_xi_maps_applies(const FieldParams &fp);

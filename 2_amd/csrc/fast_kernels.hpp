@@ -664,8 +664,8 @@ xf_fast(DemagGeom g, const float2 *__restrict__ tw, const float2 *__restrict__ r
 
 // Two neighbouring cells (k0 even, k0 + 1 < n2) of one row at once, for
 // uniform material parameters (MAPS false) or the Msat, Aex, alpha and Ku
-// maps, the B_ext and anisU masks and the thermal field (MAPS true), and no
-// spin-transfer term: the same per-cell arithmetic as cell_epilogue
+// maps, the B_ext and anisU masks, the thermal field and H_dex (MAPS true),
+// and no spin-transfer term: the same per-cell arithmetic as cell_epilogue
 // (exchange6 order dim 0, dim 2, dim 1; SumUpdater order), with 8-byte loads
 // and stores of the cell pair.
 
@@ -818,12 +818,21 @@ __device__ __forceinline__ void pair_epilogue(const DemagGeom &g, const FieldPar
             H1[c] += k1 * mu1 * u1[c];
         }
     }
-    // H_therm, the SumUpdater's last term
+    // H_therm, the SumUpdater's last term, then H_dex from its own
+    // convolution (demagexch with an Msat mask)
     if constexpr (MAPS) {
         if (fp.htherm) {
 #pragma unroll
             for (int c = 0; c < 3; c++) {
                 const float2 t = ld_off<float2>(fp.htherm + c * N, b0);
+                H0[c] += t.x;
+                H1[c] += t.y;
+            }
+        }
+        if (fp.hdex) {
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                const float2 t = ld_off<float2>(fp.hdex + c * N, b0);
                 H0[c] += t.x;
                 H1[c] += t.y;
             }
@@ -874,9 +883,9 @@ __device__ __forceinline__ void pair_epilogue(const DemagGeom &g, const FieldPar
 // xi: C2R along x (imaginary parts of DC and Nyquist ignored), keep k < n2,
 // then the fused field / torque / RK epilogue, one of three variants:
 // XI_UNIFORM the pair epilogue for uniform parameters, XI_MAPS the pair
-// epilogue with maps and masks (Msat, Aex, alpha, Ku, B_ext, anisU) and the
-// thermal field, XI_GENERAL the per-cell epilogue (cell_epilogue: all of
-// these plus Zhang-Li and demagexch).  Each is
+// epilogue with maps and masks (Msat, Aex, alpha, Ku, B_ext, anisU), the
+// thermal field and H_dex, XI_GENERAL the per-cell epilogue (cell_epilogue:
+// all of these plus the Zhang-Li torque).  Each is
 // its own kernel: the general epilogue's register demand would otherwise set
 // the occupancy of the pair variants.
 enum XiVariant { XI_UNIFORM = 0, XI_MAPS = 1, XI_GENERAL = 2 };

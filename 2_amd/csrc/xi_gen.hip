@@ -1,6 +1,6 @@
 // xi_gen.hip -- the xi pass of the power-of-two lengths with the general
-// per-cell epilogue (Zhang-Li with its polarisation and xi masks, demagexch,
-// with or without the other maps, masks and the thermal field), in a file of
+// per-cell epilogue (Zhang-Li with its polarisation and xi masks, with or
+// without the other maps, masks, the thermal field and H_dex), in a file of
 // its own so that it compiles in parallel with demag_fast.hip.
 #include "fast_launch.hpp"
 

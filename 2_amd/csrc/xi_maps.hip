@@ -1,7 +1,7 @@
 // xi_maps.hip -- the xi pass of the power-of-two lengths with the two-cell
-// epilogue for maps and masks (Msat, Aex, alpha, Ku, B_ext, anisU) and the
-// thermal field, in a file of its own so that it compiles in parallel with
-// the other passes.
+// epilogue for maps and masks (Msat, Aex, alpha, Ku, B_ext, anisU), the
+// thermal field and H_dex, in a file of its own so that it compiles in
+// parallel with the other passes.
 #include "fast_launch.hpp"
 
 namespace mmx {

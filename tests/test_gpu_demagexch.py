@@ -174,12 +174,27 @@ def test_reference_rk12_script(mmx_gpu):
     assert relerr(s.getarray("H_eff"), prob.heff(m)) < TOL
 
 
-def test_heff_with_demagexch_and_msat_mask(mmx_gpu):
+@pytest.mark.parametrize("register_x", [False, True])
+def test_heff_with_demagexch_and_msat_mask(mmx_gpu, register_x):
     """With an Msat mask the reference keeps building the dex kernel from the
     Msat multiplier and convolves m itself (mod_demagexchange.go:74-79,143),
-    while B takes Msat_map m: H_eff = H_demag(Msat_map m) + H_ex + H_dex(m)."""
+    while B takes Msat_map m: H_eff = H_demag(Msat_map m) + H_ex + H_dex(m).
+    register_x: the register-FFT x passes at this size (their two-cell
+    epilogue adds H_dex); otherwise the generic passes this grid gets by
+    default."""
+    import ctypes as C
     n, cell, pbc = (2, 16, 32), (3e-9, 4e-9, 4e-9), (0, 0, 0)
-    _say("H_eff + H_dex with an Msat mask", n)
+    _say("H_eff + H_dex with an Msat mask", n, "register x" if register_x else "default")
+    lib = mmx_gpu.lib()
+    prev = C.c_int()
+    mmx_gpu.check(lib.mmx_set_fast_x_min_blocks(0 if register_x else 256, C.byref(prev)))
+    try:
+        _heff_dex_msat_mask(mmx_gpu, n, cell, pbc)
+    finally:
+        mmx_gpu.check(lib.mmx_set_fast_x_min_blocks(prev.value, None))
+
+
+def _heff_dex_msat_mask(mmx_gpu, n, cell, pbc):
     s, m = _sim(mmx_gpu, n, cell, pbc, seed=11)
     ms = np.random.default_rng(12).uniform(0.5, 1.0, size=n).astype(np.float32)
     s.setmask("Msat", ms, 1)
