@@ -188,7 +188,10 @@ MAP_CASES = [((4, 16, 32), (0, 0, 0), "msat"), ((2, 32, 64), (0, 0, 0), "msat+ae
              ((2, 16, 64), (0, 0, 1), "all"), ((4, 16, 32), (0, 1, 0), "msat+aex"),
              ((4, 8, 16), (1, 0, 0), "msat"), ((2, 32, 32), (0, 0, 0), "all"),
              ((4, 16, 32), (0, 0, 0), "alpha+ku"), ((4, 16, 32), (0, 0, 0), "masks"),
-             ((2, 16, 64), (0, 0, 1), "all+masks"), ((4, 16, 32), (0, 1, 0), "all+masks")]
+             ((2, 16, 64), (0, 0, 1), "all+masks"), ((4, 16, 32), (0, 1, 0), "all+masks"),
+             # x lengths with a factor 3, 5, 7 (the map variant of the mixed-radix passes)
+             ((2, 16, 48), (0, 0, 0), "all+masks"), ((2, 8, 40), (0, 0, 0), "msat"),
+             ((2, 8, 56), (0, 1, 0), "msat+aex")]
 
 
 @pytest.mark.parametrize("n,pbc,maps", MAP_CASES)
