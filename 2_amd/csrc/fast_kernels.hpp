@@ -967,9 +967,29 @@ xi_fast(DemagGeom g, const float2 *__restrict__ tw, const float2 *__restrict__ r
                 hd1[c] = fp.demag_scale * v[c][q].y;
             }
             if (VAR == XI_GENERAL && MODE != EPI_B) {
+                // both cells' outputs stored as float2 pairs (two 4-byte
+                // stores per component wrote each line twice, half-filled)
                 const long I0 = (long)row * g.n2 + k0;
-                cell_epilogue<MODE>(g, fp, io, I0, i, j, k0, hd0, errmax);
-                cell_epilogue<MODE>(g, fp, io, I0 + 1, i, j, k0 + 1, hd1, errmax);
+                CellOut o0, o1;
+                cell_epilogue<MODE>(g, fp, io, I0, i, j, k0, hd0, errmax, &o0);
+                cell_epilogue<MODE>(g, fp, io, I0 + 1, i, j, k0 + 1, hd1, errmax, &o1);
+                const size_t N = g.N;
+                const unsigned b0 = (unsigned)I0 * 4u;
+                if (io.hout) {
+#pragma unroll
+                    for (int c = 0; c < 3; c++)
+                        st_off<float2>(io.hout + c * N, b0, float2{o0.h[c], o1.h[c]});
+                }
+                if (io.tout) {
+#pragma unroll
+                    for (int c = 0; c < 3; c++)
+                        st_off<float2>(io.tout + c * N, b0, float2{o0.t[c], o1.t[c]});
+                }
+                if (MODE == EPI_CORR) {
+#pragma unroll
+                    for (int c = 0; c < 3; c++)
+                        st_off<float2>(io.mnew + c * N, b0, float2{o0.mn[c], o1.mn[c]});
+                }
             } else {
                 pair_epilogue<MODE, VAR == XI_MAPS>(g, fp, io, (unsigned)(row * g.n2 + k0), i, j,
                                                     k0, hd0, hd1, errmax);
