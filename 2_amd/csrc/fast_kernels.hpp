@@ -967,8 +967,8 @@ xi_fast(DemagGeom g, const float2 *__restrict__ tw, const float2 *__restrict__ r
                 hd1[c] = fp.demag_scale * v[c][q].y;
             }
             if (VAR == XI_GENERAL && MODE != EPI_B) {
-                // both cells' outputs stored as float2 pairs (two 4-byte
-                // stores per component wrote each line twice, half-filled)
+                // both cells' outputs stored as float2 pairs, as the
+                // two-cell variants store them
                 const long I0 = (long)row * g.n2 + k0;
                 CellOut o0, o1;
                 cell_epilogue<MODE>(g, fp, io, I0, i, j, k0, hd0, errmax, &o0);
